@@ -1,0 +1,260 @@
+/*
+ * kz.h — C ABI of the MI355X embedding-extension engine (libkz_hip.so).
+ *
+ * This is the drop-in boundary between host code (the Khuzdul application
+ * API in openkhuzdulgraph_amd/khuzdul, the Python host mirror, or any FFI
+ * caller) and the gfx950 HIP kernels.  Signatures use plain pointers, sizes
+ * and integer handles only.  Every function returns an int: 0 on success, a
+ * negative KZ_ERR_* code on failure; kz_last_error() then returns a message
+ * for the calling thread.  No C++ exception crosses this boundary.
+ *
+ * Ownership: host buffers passed in are caller-owned and only read during
+ * the call; kz_csr arrays returned by the library are released with
+ * kz_csr_release(); device memory behind kz_graph / kz_plan handles is
+ * library-owned and released with the matching *_release call.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the OpenKhuzdulGraph source tree).
+ */
+#ifndef KZ_H
+#define KZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KZ_OK 0
+#define KZ_ERR_INVALID (-1)     /* bad argument or malformed input           */
+#define KZ_ERR_NOMEM (-2)       /* host or device allocation failed          */
+#define KZ_ERR_IO (-3)          /* file could not be read or written         */
+#define KZ_ERR_DEVICE (-4)      /* HIP runtime error / no usable device      */
+#define KZ_ERR_UNSUPPORTED (-5) /* plan or pattern outside what is supported */
+#define KZ_ERR_BUDGET (-6)      /* request exceeds the configured HBM budget */
+#define KZ_ERR_STATE (-7)       /* call made in the wrong state (e.g. no init) */
+
+/* ------------------------------------------------------------------------ */
+/* Host-side CSR graph (row a10: graph storage + .dgraph loader)            */
+/* ------------------------------------------------------------------------ */
+
+/* Compressed sparse rows with 64-bit row offsets and 32-bit column ids.
+ * Replaces HashBasedNumaAwareDistributedCSRGraph (include/graph.h:308-425)
+ * on one device: the whole graph, not a hash partition.  Neighbour lists are
+ * strictly increasing.  `oriented` is 1 when every stored edge u->v has
+ * v > u (degree-ordered rank ids, toolkits/to_oriented_dataset_converter.cc). */
+typedef struct kz_csr {
+    uint32_t num_vertices;
+    uint32_t max_degree;
+    uint64_t num_entries; /* stored directed entries = .dgraph num_edges   */
+    uint32_t oriented;
+    uint32_t has_labels;
+    uint64_t* rowptr; /* num_vertices + 1                                 */
+    uint32_t* col;    /* num_entries                                      */
+    uint32_t* labels; /* num_vertices, or NULL                            */
+} kz_csr;
+
+/* Read a .dgraph file: packed 20-byte header {u32 nv; u64 ne; i32
+ * has_node_label; i32 has_edge_label}, optional u32 labels[nv], then ne
+ * packed {u32 src, u32 dst} sorted by (src, dst).
+ * Replaces JointHashBasedNumaAwareDistGraphLoader::load_graphs
+ * (include/graph_loader.h:706-961) and its format (graph_loader.h:31-42).
+ * `oriented` is detected (every dst > src).  Edge labels are rejected, as in
+ * the reference (graph_loader.h:770). */
+int kz_dgraph_load(const char* path, kz_csr* out);
+
+/* Write a CSR in the same format (toolkits/converter.cc:178-263 for
+ * symmetric graphs, to_oriented_dataset_converter.cc:66-117 for oriented). */
+int kz_dgraph_store(const char* path, const kz_csr* g);
+
+/* Free arrays owned by a kz_csr returned by this library; zeroes *g. */
+void kz_csr_release(kz_csr* g);
+
+/* Build a clean symmetric CSR from an undirected edge list: drop self-loops
+ * and duplicates, drop isolated vertices and relabel the rest 0..n'-1 in
+ * ascending original id (datasets/live_journal/standardize.cc:66-114), store
+ * both directions sorted (toolkits/converter.cc:237-263).  Ids must be
+ * < num_vertices_hint. */
+int kz_csr_from_edges(const uint32_t* src, const uint32_t* dst, uint64_t num_edges,
+                      uint32_t num_vertices_hint, kz_csr* out);
+
+/* Seeded synthetic graphs, cleaned as kz_csr_from_edges.  All randomness is
+ * a counter-based splitmix64 stream, so the bytes are identical on every
+ * host and thread count. */
+int kz_gen_erdos_renyi(uint32_t n, uint64_t m, uint64_t seed, kz_csr* out);
+int kz_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                uint64_t seed, kz_csr* out);
+int kz_gen_chung_lu(uint32_t n, uint64_t num_draws, double exponent, uint32_t max_weight,
+                    uint64_t seed, kz_csr* out);
+
+/* Degree orientation: rank vertices by (degree asc, id asc), relabel by rank,
+ * keep u->v iff rank(v) > rank(u) (toolkits/to_oriented_dataset_converter.cc:
+ * 27-124).  `sym` must be symmetric. */
+int kz_orient_by_degree(const kz_csr* sym, kz_csr* out);
+
+/* Structural checks of toolkits/check_graph.cc:44-68: strictly increasing
+ * lists, no self-loops, degree sum = entries; symmetric graphs also need
+ * every reverse edge, oriented graphs every dst > src.  Returns KZ_OK or
+ * KZ_ERR_INVALID with a message. */
+int kz_csr_check(const kz_csr* g);
+
+/* FNV-1a 64-bit digest of the CSR arrays (used to prove that two hosts
+ * rebuilt the same graph). */
+uint64_t kz_csr_digest(const kz_csr* g);
+
+/* ------------------------------------------------------------------------ */
+/* Device runtime                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Select the HIP device and create the library's stream.  Replaces
+ * DistributedSys::init_distributed_sys (src/distributed_sys.cc:23-26) for the
+ * device side.  Safe to call again with the same device. */
+int kz_init(int device);
+int kz_shutdown(void);
+const char* kz_last_error(void);
+
+typedef struct kz_device_info {
+    char name[64];
+    char arch[32];
+    int compute_units;
+    int warp_size;
+    uint64_t total_mem;
+    uint64_t lds_per_block;
+    int clock_khz;
+    int mem_clock_khz;
+    int mem_bus_width;
+    int l2_bytes;
+} kz_device_info;
+int kz_get_device_info(kz_device_info* out);
+
+/* Opaque handle to a CSR resident in HBM. */
+typedef struct kz_graph_s* kz_graph;
+
+/* Copy a host CSR into HBM (row offsets u64, columns u32, out-degree u32).
+ * Replaces the per-socket CSR build of graph_loader.h:873-901. */
+int kz_graph_upload(const kz_csr* g, kz_graph* out);
+int kz_graph_release(kz_graph g);
+int kz_graph_num_vertices(kz_graph g, uint32_t* out);
+
+/* ------------------------------------------------------------------------ */
+/* Counting                                                                  */
+/* ------------------------------------------------------------------------ */
+
+#define KZ_MAX_PATTERN 5     /* MAX_PATTERN_SIZE (include/engine.h:23)        */
+#define KZ_MAX_AGGREGATORS 8
+#define KZ_MAX_OPS 4         /* set operations per (size, order type)         */
+#define KZ_MAX_OUTS 4        /* scatter / count outputs per (size, order type) */
+#define KZ_MAX_STEP_LISTS 128
+
+#define KZ_METHOD_PLAIN 0      /* level-by-level: materialised prefixes and
+                                  embedding queues, chunked by an HBM budget
+                                  (src/engine.cc:914-1118)                    */
+#define KZ_METHOD_PERSISTENT 1 /* degree-bucketed persistent kernels over a
+                                  device work queue, one hipGraph launch      */
+
+typedef struct kz_run_stats {
+    double device_ms;                   /* HIP-event time on the library stream */
+    double wall_ms;                     /* host wall time of the call           */
+    uint64_t launches;                  /* kernel or graph launches             */
+    uint64_t tasks;                     /* root tasks (persistent form)         */
+    uint64_t embeddings[KZ_MAX_PATTERN + 1]; /* embeddings extended per size    */
+    uint64_t chunks;                    /* HBM-budget chunks (plain form)       */
+    uint64_t peak_bytes;                /* peak device scratch of the call      */
+} kz_run_stats;
+
+/* Exact k-clique count (k = 3..5).  Equals the count printed by
+ * kautomine/{triangle,four_clique,five_clique}.cc and the kgraphpi apps of
+ * the same names on the same .dgraph.  The persistent form needs every
+ * forward out-degree <= 8192 (true after degree orientation). */
+int kz_count_cliques(kz_graph g, int k, int method, uint64_t* count, kz_run_stats* stats);
+
+/* Exact terms of the merge-model byte count of the clique schedules on the
+ * forward (v > u) graph, for throughput reporting:
+ *   out[0] = m              out[1] = T = sum_{(u,v)} c(u,v)
+ *   out[2] = sum c(u,v)^2   out[3] = sum_{(u,v,w) in triangles} d+(w)
+ *   out[4] = sum_u d+(u)^2  out[5] = sum_{(u,v)} d+(v)
+ * with c(u,v) = |N+(u) & N+(v)|.  Then B3 = 4(m + out[4] + out[5]) and
+ * B4 = B3 + 4(T + out[2] + out[3]) bytes.  No reference counterpart: the
+ * reference reports no bandwidth for this path. */
+int kz_clique_workload(kz_graph g, uint64_t* out6);
+
+/* ---- Plan IR: a generated apply() schedule in data form ----------------
+ * Replaces the ApplyFunction callback (include/engine.h:589) with a table the
+ * GPU executes.  An embedding of size s (matched vertices m_0..m_{s-1}) is
+ * extended by the step list selected by its ORDER CODE: with r_j the dense
+ * rank of m_j among the distinct matched values, code = sum_j r_j * s^j.
+ * A step list runs up to KZ_MAX_OPS set operations (intersect_with /
+ * subtracted_by, src/vertex_set.cc:20-86) whose result j is SLOT(s, j), then
+ * up to KZ_MAX_OUTS outputs that iterate a set: SCATTER makes a child of
+ * size s+1 per passing candidate (EmbeddingExplorationEngine::scatter,
+ * src/engine.cc:1385-1453), COUNT adds the number of passing candidates to an
+ * aggregator (Aggregator::add, include/engine.h:649-651).
+ * A candidate v passes when bit p of `pred` is set, where its POSITION CLASS
+ * p = 2 * |{distinct matched values < v}| + [v equals a matched value]. */
+#define KZ_SET_NONE 0
+#define KZ_SET_NBR 1  /* N(m_a)                                          */
+#define KZ_SET_SLOT 2 /* slot b computed by the step list at size a <= s */
+typedef struct kz_set_ref {
+    uint8_t kind, a, b, reserved;
+} kz_set_ref;
+
+#define KZ_OP_INTERSECT 1
+#define KZ_OP_SUBTRACT 2
+typedef struct kz_op {
+    uint8_t op, reserved[3];
+    kz_set_ref lhs, rhs;
+} kz_op;
+
+#define KZ_OUT_SCATTER 1
+#define KZ_OUT_COUNT 2
+typedef struct kz_out {
+    uint8_t kind;
+    uint8_t aggregator;   /* COUNT target                                  */
+    uint8_t child_active; /* SCATTER: is_new_v_active (informational)      */
+    uint8_t finalized;    /* SCATTER: is_finalized (informational)         */
+    kz_set_ref src;
+    uint32_t pred;
+} kz_out;
+
+typedef struct kz_step_list {
+    uint8_t num_ops, num_outs, reserved[2];
+    kz_op ops[KZ_MAX_OPS];
+    kz_out outs[KZ_MAX_OUTS];
+} kz_step_list;
+
+typedef struct kz_plan_desc {
+    uint32_t pattern_size;    /* K: embeddings of size 1..K-1 are extended */
+    uint32_t num_aggregators;
+    uint32_t multiplicity[KZ_MAX_AGGREGATORS]; /* informational: printed count = raw / multiplicity */
+    uint32_t num_step_lists;
+    kz_step_list lists[KZ_MAX_STEP_LISTS];
+    /* type_table[s][code] = index into lists, or 0xFF if that order type is
+     * never reached.  Codes are < s^s <= 256. */
+    uint8_t type_table[KZ_MAX_PATTERN][256];
+} kz_plan_desc;
+
+/* Built-in plans: the kautomine schedules of triangle.cc (k=3),
+ * four_clique.cc (k=4), five_clique.cc (k=5). */
+int kz_plan_clique(int k, kz_plan_desc* out);
+
+/* Check a plan for structural validity (references to slots that exist and
+ * are computed before use, sizes in range, last size has no scatter). */
+int kz_plan_validate(const kz_plan_desc* plan);
+
+/* Execute a plan with the plain level-by-level form.  raw_counts[a] gets the
+ * uint64 sum of aggregator a (mod 2^64), before multiplicity division, as
+ * Aggregator<EdgeId>::evaluate returns it (include/engine.h:628-645). */
+int kz_count_plan(kz_graph g, const kz_plan_desc* plan, uint64_t* raw_counts, uint32_t num_counts,
+                  kz_run_stats* stats);
+
+/* Per-level HBM budget of the plain form, in bytes (the analogue of
+ * memory_size_per_level, src/distributed_application.cc:20-22).  A smaller
+ * budget forces more chunks; results do not change. */
+int kz_set_level_budget(uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KZ_H */

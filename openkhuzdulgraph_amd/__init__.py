@@ -1,0 +1,199 @@
+"""openkhuzdulgraph_amd — MI355X-native embedding-extension engine.
+
+Python host mirror of the C ABI in include/kz.h.  Everything here calls
+libkz_hip.so; there is no Python or CPU compute path, and a missing library
+raises KzLibraryMissing.
+
+    import openkhuzdulgraph_amd as kz
+    g = kz.gen_rmat(scale=18, edge_factor=16, seed=1).oriented()
+    kz.init(0)
+    dg = g.to_device()
+    n4 = dg.count_cliques(4, method="persistent")
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import KzError, KzLibraryMissing, check, lib
+
+__all__ = [
+    "CSRGraph", "DeviceGraph", "KzError", "KzLibraryMissing", "init", "shutdown", "device_info",
+    "gen_erdos_renyi", "gen_rmat", "gen_chung_lu", "from_edges", "load_dgraph", "set_level_budget",
+    "clique_plan",
+]
+
+_METHODS = {"plain": _ffi.KZ_METHOD_PLAIN, "persistent": _ffi.KZ_METHOD_PERSISTENT}
+
+
+class CSRGraph:
+    """Host CSR owned by libkz_hip.so (kz_csr).  Arrays are exposed as
+    zero-copy numpy views valid while this object lives."""
+
+    def __init__(self, raw: _ffi.kz_csr):
+        self._raw = raw
+
+    def __del__(self):
+        try:
+            if self._raw.rowptr:
+                lib().kz_csr_release(C.byref(self._raw))
+        except Exception:
+            pass
+
+    @property
+    def num_vertices(self) -> int:
+        return int(self._raw.num_vertices)
+
+    @property
+    def num_entries(self) -> int:
+        return int(self._raw.num_entries)
+
+    @property
+    def max_degree(self) -> int:
+        return int(self._raw.max_degree)
+
+    @property
+    def oriented_flag(self) -> bool:
+        return bool(self._raw.oriented)
+
+    @property
+    def rowptr(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._raw.rowptr, (self.num_vertices + 1,))
+
+    @property
+    def col(self) -> np.ndarray:
+        if self.num_entries == 0:
+            return np.zeros(0, np.uint32)
+        return np.ctypeslib.as_array(self._raw.col, (self.num_entries,))
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.rowptr).astype(np.uint32)
+
+    def oriented(self) -> "CSRGraph":
+        """Degree orientation (to_oriented_dataset_converter.cc:27-124)."""
+        out = _ffi.kz_csr()
+        check(lib().kz_orient_by_degree(C.byref(self._raw), C.byref(out)))
+        return CSRGraph(out)
+
+    def check(self) -> None:
+        check(lib().kz_csr_check(C.byref(self._raw)))
+
+    def digest(self) -> int:
+        return int(lib().kz_csr_digest(C.byref(self._raw)))
+
+    def store(self, path: str) -> None:
+        check(lib().kz_dgraph_store(str(path).encode(), C.byref(self._raw)))
+
+    def to_device(self) -> "DeviceGraph":
+        h = C.c_void_p()
+        check(lib().kz_graph_upload(C.byref(self._raw), C.byref(h)))
+        return DeviceGraph(h, self)
+
+    def __repr__(self) -> str:
+        kind = "oriented" if self.oriented_flag else "symmetric"
+        return f"CSRGraph(n={self.num_vertices}, entries={self.num_entries}, max_degree={self.max_degree}, {kind})"
+
+
+def _new(fn, *args) -> CSRGraph:
+    out = _ffi.kz_csr()
+    check(fn(*args, C.byref(out)))
+    return CSRGraph(out)
+
+
+def gen_erdos_renyi(n: int, m: int, seed: int) -> CSRGraph:
+    return _new(lib().kz_gen_erdos_renyi, n, m, seed)
+
+
+def gen_rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1) -> CSRGraph:
+    return _new(lib().kz_gen_rmat, scale, edge_factor, a, b, c, seed)
+
+
+def gen_chung_lu(n: int, num_draws: int, exponent: float, max_weight: int, seed: int) -> CSRGraph:
+    return _new(lib().kz_gen_chung_lu, n, num_draws, exponent, max_weight, seed)
+
+
+def from_edges(src, dst, num_vertices_hint: int) -> CSRGraph:
+    s = np.ascontiguousarray(src, dtype=np.uint32)
+    d = np.ascontiguousarray(dst, dtype=np.uint32)
+    if s.shape != d.shape:
+        raise ValueError("src and dst differ in length")
+    P = C.POINTER(C.c_uint32)
+    return _new(lib().kz_csr_from_edges, s.ctypes.data_as(P), d.ctypes.data_as(P), len(s), num_vertices_hint)
+
+
+def load_dgraph(path: str) -> CSRGraph:
+    return _new(lib().kz_dgraph_load, str(path).encode())
+
+
+def init(device: int = 0) -> None:
+    check(lib().kz_init(device))
+
+
+def shutdown() -> None:
+    check(lib().kz_shutdown())
+
+
+def device_info() -> dict:
+    info = _ffi.kz_device_info()
+    check(lib().kz_get_device_info(C.byref(info)))
+    return {k: (getattr(info, k).decode() if isinstance(getattr(info, k), bytes) else getattr(info, k))
+            for k, _ in info._fields_}
+
+
+def set_level_budget(nbytes: int) -> None:
+    check(lib().kz_set_level_budget(int(nbytes)))
+
+
+def clique_plan(k: int) -> _ffi.kz_plan_desc:
+    p = _ffi.kz_plan_desc()
+    check(lib().kz_plan_clique(k, C.byref(p)))
+    return p
+
+
+class DeviceGraph:
+    """A CSR resident in HBM (kz_graph)."""
+
+    def __init__(self, handle: C.c_void_p, host: Optional[CSRGraph] = None):
+        self._h = handle
+        self.host = host
+        self.last_stats: dict = {}
+
+    def release(self) -> None:
+        if self._h:
+            check(lib().kz_graph_release(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def count_cliques(self, k: int, method: str = "persistent") -> int:
+        out = C.c_uint64()
+        st = _ffi.kz_run_stats()
+        check(lib().kz_count_cliques(self._h, k, _METHODS[method], C.byref(out), C.byref(st)))
+        self.last_stats = st.as_dict()
+        return int(out.value)
+
+    def clique_workload(self) -> dict:
+        """Merge-model byte terms (kz_clique_workload) and B3/B4 in bytes."""
+        out = (C.c_uint64 * 6)()
+        check(lib().kz_clique_workload(self._h, out))
+        m, T, c2, sw, dsq, dv = (int(x) for x in out)
+        b3 = 4 * (m + dsq + dv)
+        return {"m": m, "triangles": T, "sum_c2": c2, "sum_tri_dw": sw, "sum_d2": dsq, "sum_dv": dv,
+                "B3": b3, "B4": b3 + 4 * (T + c2 + sw)}
+
+    def count_plan(self, plan: _ffi.kz_plan_desc) -> list[int]:
+        n = max(1, int(plan.num_aggregators))
+        out = (C.c_uint64 * _ffi.KZ_MAX_AGGREGATORS)()
+        st = _ffi.kz_run_stats()
+        check(lib().kz_count_plan(self._h, C.byref(plan), out, _ffi.KZ_MAX_AGGREGATORS, C.byref(st)))
+        self.last_stats = st.as_dict()
+        return [int(out[i]) for i in range(int(plan.num_aggregators))][:n]

@@ -1,0 +1,197 @@
+"""ctypes binding of include/kz.h (libkz_hip.so).
+
+The library is loaded from openkhuzdulgraph_amd/lib/ only; there is no
+fallback implementation: a missing library raises ImportError-like
+KzLibraryMissing at first use.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libkz_hip.so"
+
+KZ_OK = 0
+KZ_ERR_INVALID = -1
+KZ_ERR_NOMEM = -2
+KZ_ERR_IO = -3
+KZ_ERR_DEVICE = -4
+KZ_ERR_UNSUPPORTED = -5
+KZ_ERR_BUDGET = -6
+KZ_ERR_STATE = -7
+
+KZ_MAX_PATTERN = 5
+KZ_MAX_AGGREGATORS = 8
+KZ_MAX_OPS = 4
+KZ_MAX_OUTS = 4
+KZ_MAX_STEP_LISTS = 128
+
+KZ_METHOD_PLAIN = 0
+KZ_METHOD_PERSISTENT = 1
+
+KZ_SET_NONE = 0
+KZ_SET_NBR = 1
+KZ_SET_SLOT = 2
+KZ_OP_INTERSECT = 1
+KZ_OP_SUBTRACT = 2
+KZ_OUT_SCATTER = 1
+KZ_OUT_COUNT = 2
+
+
+class KzLibraryMissing(RuntimeError):
+    """libkz_hip.so has not been built (python -m openkhuzdulgraph_amd.build)."""
+
+
+class KzError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"kz error {code}: {msg}")
+        self.code = code
+
+
+class kz_csr(C.Structure):
+    _fields_ = [
+        ("num_vertices", C.c_uint32),
+        ("max_degree", C.c_uint32),
+        ("num_entries", C.c_uint64),
+        ("oriented", C.c_uint32),
+        ("has_labels", C.c_uint32),
+        ("rowptr", C.POINTER(C.c_uint64)),
+        ("col", C.POINTER(C.c_uint32)),
+        ("labels", C.POINTER(C.c_uint32)),
+    ]
+
+
+class kz_device_info(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 64),
+        ("arch", C.c_char * 32),
+        ("compute_units", C.c_int),
+        ("warp_size", C.c_int),
+        ("total_mem", C.c_uint64),
+        ("lds_per_block", C.c_uint64),
+        ("clock_khz", C.c_int),
+        ("mem_clock_khz", C.c_int),
+        ("mem_bus_width", C.c_int),
+        ("l2_bytes", C.c_int),
+    ]
+
+
+class kz_run_stats(C.Structure):
+    _fields_ = [
+        ("device_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("launches", C.c_uint64),
+        ("tasks", C.c_uint64),
+        ("embeddings", C.c_uint64 * (KZ_MAX_PATTERN + 1)),
+        ("chunks", C.c_uint64),
+        ("peak_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {
+            "device_ms": self.device_ms,
+            "wall_ms": self.wall_ms,
+            "launches": int(self.launches),
+            "tasks": int(self.tasks),
+            "embeddings": [int(x) for x in self.embeddings],
+            "chunks": int(self.chunks),
+            "peak_bytes": int(self.peak_bytes),
+        }
+
+
+class kz_set_ref(C.Structure):
+    _fields_ = [("kind", C.c_uint8), ("a", C.c_uint8), ("b", C.c_uint8), ("reserved", C.c_uint8)]
+
+
+class kz_op(C.Structure):
+    _fields_ = [("op", C.c_uint8), ("reserved", C.c_uint8 * 3), ("lhs", kz_set_ref), ("rhs", kz_set_ref)]
+
+
+class kz_out(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint8),
+        ("aggregator", C.c_uint8),
+        ("child_active", C.c_uint8),
+        ("finalized", C.c_uint8),
+        ("src", kz_set_ref),
+        ("pred", C.c_uint32),
+    ]
+
+
+class kz_step_list(C.Structure):
+    _fields_ = [
+        ("num_ops", C.c_uint8),
+        ("num_outs", C.c_uint8),
+        ("reserved", C.c_uint8 * 2),
+        ("ops", kz_op * KZ_MAX_OPS),
+        ("outs", kz_out * KZ_MAX_OUTS),
+    ]
+
+
+class kz_plan_desc(C.Structure):
+    _fields_ = [
+        ("pattern_size", C.c_uint32),
+        ("num_aggregators", C.c_uint32),
+        ("multiplicity", C.c_uint32 * KZ_MAX_AGGREGATORS),
+        ("num_step_lists", C.c_uint32),
+        ("lists", kz_step_list * KZ_MAX_STEP_LISTS),
+        ("type_table", (C.c_uint8 * 256) * KZ_MAX_PATTERN),
+    ]
+
+
+_lib = None
+
+_SIGS = {
+    "kz_last_error": (C.c_char_p, []),
+    "kz_dgraph_load": (C.c_int, [C.c_char_p, C.POINTER(kz_csr)]),
+    "kz_dgraph_store": (C.c_int, [C.c_char_p, C.POINTER(kz_csr)]),
+    "kz_csr_release": (None, [C.POINTER(kz_csr)]),
+    "kz_csr_from_edges": (C.c_int, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint64, C.c_uint32,
+                                    C.POINTER(kz_csr)]),
+    "kz_gen_erdos_renyi": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, C.POINTER(kz_csr)]),
+    "kz_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                              C.POINTER(kz_csr)]),
+    "kz_gen_chung_lu": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_uint32, C.c_uint64, C.POINTER(kz_csr)]),
+    "kz_orient_by_degree": (C.c_int, [C.POINTER(kz_csr), C.POINTER(kz_csr)]),
+    "kz_csr_check": (C.c_int, [C.POINTER(kz_csr)]),
+    "kz_csr_digest": (C.c_uint64, [C.POINTER(kz_csr)]),
+    "kz_init": (C.c_int, [C.c_int]),
+    "kz_shutdown": (C.c_int, []),
+    "kz_get_device_info": (C.c_int, [C.POINTER(kz_device_info)]),
+    "kz_graph_upload": (C.c_int, [C.POINTER(kz_csr), C.POINTER(C.c_void_p)]),
+    "kz_graph_release": (C.c_int, [C.c_void_p]),
+    "kz_graph_num_vertices": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "kz_count_cliques": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_uint64), C.POINTER(kz_run_stats)]),
+    "kz_plan_clique": (C.c_int, [C.c_int, C.POINTER(kz_plan_desc)]),
+    "kz_plan_validate": (C.c_int, [C.POINTER(kz_plan_desc)]),
+    "kz_count_plan": (C.c_int, [C.c_void_p, C.POINTER(kz_plan_desc), C.POINTER(C.c_uint64), C.c_uint32,
+                                C.POINTER(kz_run_stats)]),
+    "kz_set_level_budget": (C.c_int, [C.c_uint64]),
+    "kz_clique_workload": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libkz_hip.so (once) and bind every declared symbol."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise KzLibraryMissing(f"{LIB_PATH} not built; run python -m openkhuzdulgraph_amd.build")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def check(rc: int) -> None:
+    if rc != KZ_OK:
+        msg = lib().kz_last_error()
+        raise KzError(rc, msg.decode() if msg else "")

@@ -1,0 +1,70 @@
+// Device-side shared declarations for libkz_hip.so (gfx950 / CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "kz_internal.h"
+
+#define KZ_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess)                                                          \
+            KZ_FAIL(KZ_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                    __FILE__, __LINE__);                                               \
+    } while (0)
+
+namespace kz {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// Library-wide device state (one device per process).
+struct Runtime {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    size_t lds_per_block = 0;
+    bool ready = false;
+};
+Runtime& rt();
+int ensure_ready();
+
+// A CSR resident in HBM.  rowptr is u64 (graphs beyond 2^32 entries),
+// col and deg are u32.  `vbucket` holds vertex ids grouped by out-degree
+// bucket for the bitmap kernels (built lazily, see kz_clique.hip).
+struct DeviceGraph {
+    uint32_t n = 0;
+    uint64_t nnz = 0;
+    uint32_t max_degree = 0;
+    int oriented = 0;
+    uint64_t* rowptr = nullptr;
+    uint32_t* col = nullptr;
+    uint32_t* deg = nullptr;
+    // host copies of the degree array (for bucketing and plan sizing)
+    std::vector<uint32_t> h_deg;
+    uint64_t bytes = 0;
+    // forward part (v > u) of a symmetric graph, built on first use by the
+    // clique kernels; for oriented graphs these alias rowptr/col.
+    uint64_t* fwd_rowptr = nullptr;
+    uint32_t* fwd_col = nullptr;
+    std::vector<uint32_t> h_fdeg;
+    bool fwd_ready = false;
+    // prepared persistent clique executions, indexed by k (3..5)
+    std::shared_ptr<void> clique_exec[6];
+};
+
+// Device scratch arena that grows on demand and is reused across calls.
+struct Scratch {
+    void* ptr = nullptr;
+    size_t size = 0;
+};
+int scratch_reserve(Scratch& s, size_t bytes);
+
+}  // namespace kz
+
+struct kz_graph_s {
+    kz::DeviceGraph g;
+};

@@ -1,0 +1,770 @@
+// Plain level-by-level executor of the plan IR (the "plain GPU form").
+//
+// Reference behaviour restated for the GPU:
+//   * EmbeddingExplorationEngine::extend_embeddings (src/engine.cc:914-1118)
+//     extends every embedding of one size, writing children into the next
+//     level's queue; when that queue or its buffers are full the engine dives
+//     one level deeper before continuing (src/engine.cc:1047-1082).  Here a
+//     level is processed in two passes over its embeddings: pass A sizes every
+//     embedding's children and cached prefixes (and adds COUNT outputs);
+//     after an exclusive scan the parents are cut into chunks whose children
+//     and prefixes fit the per-level HBM budget, and for each chunk pass B
+//     writes them and the next level runs before the following chunk;
+//   * an embedding is a record {parent index, new vertex}; the matched
+//     vertices are rebuilt by walking parents (CompactExtendableEmbedding,
+//     include/engine.h:795-805 and src/engine.cc:1008-1021);
+//   * cached objects (include/engine.h:592-613) become per-slot arrays of
+//     {offset, length} into a level's prefix buffer;
+//   * the set operations are those of src/vertex_set.cc:20-86, and COUNT /
+//     SCATTER outputs are the apps' candidate loops.
+// MI355X mapping: one wave64 per embedding (persistent waves pulling batches
+// from an atomic cursor); the shorter operand of an intersection (or the
+// subtrahend) is staged in an LDS hash and the other operand is streamed
+// with coalesced loads; heavily skewed pairs binary-search the long list
+// instead.  Results are compacted in order with ballot/popcount, so every
+// prefix is sorted exactly like the reference's merge output.
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "kz_device.h"
+#include "kz_kernels.h"
+#include "kz_plan.h"
+
+namespace kz {
+
+uint64_t g_level_budget = uint64_t(4) << 30;  // memory_size_per_level default (4 GiB)
+
+constexpr uint32_t kHashCap = 2048;  // per-wave LDS hash slots
+constexpr int kWavesPerBlock = 4;
+constexpr uint32_t kBatch = 8;  // embeddings per cursor grab
+
+struct DevLevel {
+    uint64_t n = 0;
+    const uint32_t* parent = nullptr;
+    const uint32_t* vertex = nullptr;
+    uint32_t root_base = 0;
+    const uint64_t* slot_off[KZ_MAX_OPS] = {};
+    const uint32_t* slot_len[KZ_MAX_OPS] = {};
+    const uint32_t* slot_data[KZ_MAX_OPS] = {};
+    uint64_t slot_base[KZ_MAX_OPS] = {};
+};
+
+struct LevelArgs {
+    DevLevel lv[KZ_MAX_PATTERN + 1];
+    int s;
+    int mode;  // 0 = pass A (sizes + counts), 1 = pass B (write)
+    uint64_t begin, end;
+    const uint64_t* rowptr;
+    const uint32_t* col;
+    const kz_step_list* lists;
+    const uint8_t* types;  // type_table row for size s
+    uint32_t* cursor;
+    unsigned long long* agg;
+    uint32_t* err;
+    // pass A outputs (indexed e - begin)
+    uint32_t* nchild;
+    uint32_t* slot_len_out[KZ_MAX_OPS];
+    // pass B outputs
+    const uint64_t* child_off;  // exclusive prefix, indexed e - begin... absolute prefix
+    uint64_t child_base;
+    uint32_t* child_parent;
+    uint32_t* child_vertex;
+    const uint64_t* slot_off_out[KZ_MAX_OPS];
+    uint64_t slot_base_out[KZ_MAX_OPS];
+    uint32_t* slot_data_out[KZ_MAX_OPS];
+    uint32_t cached_mask;
+    uint32_t* temp;
+    uint32_t temp_cap;
+};
+
+struct SetView {
+    const uint32_t* p;
+    uint32_t n;
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const uint32_t lane = threadIdx.x & 63;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Position class of v against sorted distinct matched values z[0..q).
+__device__ __forceinline__ uint32_t pos_class(uint32_t v, const uint32_t* z, int q) {
+    uint32_t lt = 0, eq = 0;
+    for (int i = 0; i < q; ++i) {
+        lt += z[i] < v;
+        eq |= z[i] == v;
+    }
+    return 2 * lt + eq;
+}
+
+// Wave-cooperative A op B.  Writes the ordered result to `out` (global) and
+// returns its length.  `hk` is this wave's LDS hash (kHashCap slots).
+__device__ uint32_t wave_set_op(int op, SetView A, SetView B, uint32_t* __restrict__ out, uint32_t* hk) {
+    const uint32_t lane = threadIdx.x & 63;
+    if (op == KZ_OP_INTERSECT && (A.n == 0 || B.n == 0)) return 0;
+    if (op == KZ_OP_SUBTRACT && A.n == 0) return 0;
+    // Choose the streamed list S and the probed list P.
+    SetView S, P;
+    if (op == KZ_OP_SUBTRACT) {
+        S = A;
+        P = B;
+    } else if (A.n <= B.n) {
+        S = B;
+        P = A;
+    } else {
+        S = A;
+        P = B;
+    }
+    // For an intersection with a very short P, stream P and search S instead.
+    bool search_mode;
+    if (op == KZ_OP_INTERSECT) {
+        if (uint64_t(S.n) > 32ull * P.n || P.n > kHashCap / 2) {
+            SetView t = S;  // stream the shorter, search the longer
+            S = P;
+            P = t;
+            search_mode = true;
+        } else {
+            search_mode = false;
+        }
+    } else {
+        search_mode = (P.n > kHashCap / 2) || (uint64_t(P.n) > 32ull * S.n);
+    }
+    uint32_t hmask = 0, hshift = 0;
+    if (!search_mode && P.n) {
+        uint32_t H = 64;
+        while (H < 2 * P.n) H <<= 1;
+        hmask = H - 1;
+        uint32_t bits = 0;
+        while ((1u << bits) < H) ++bits;
+        hshift = 32 - bits;
+        for (uint32_t i = lane; i < H; i += 64) hk[i] = kEmpty;
+        wave_lds_sync();
+        for (uint32_t i = lane; i < P.n; i += 64) {
+            const uint32_t x = P.p[i];
+            uint32_t h = (x * 2654435761u) >> hshift;
+            while (atomicCAS(&hk[h], kEmpty, x) != kEmpty) h = (h + 1) & hmask;
+        }
+        wave_lds_sync();
+    }
+    uint32_t k = 0;
+    for (uint32_t base = 0; base < S.n; base += 64) {
+        const uint32_t i = base + lane;
+        const bool valid = i < S.n;
+        const uint32_t x = valid ? S.p[i] : 0;
+        bool found = false;
+        if (valid) {
+            if (search_mode) {
+                found = bsearch_global(P.p, P.n, x);
+            } else if (P.n) {
+                uint32_t h = (x * 2654435761u) >> hshift;
+                while (true) {
+                    const uint32_t key = hk[h];
+                    if (key == x) {
+                        found = true;
+                        break;
+                    }
+                    if (key == kEmpty) break;
+                    h = (h + 1) & hmask;
+                }
+            }
+        }
+        const bool keep = valid && (op == KZ_OP_INTERSECT ? found : !found);
+        const uint64_t m = __ballot(keep);
+        if (keep) out[k + __popcll(m & lanemask_lt())] = x;
+        k += __popcll(m);
+    }
+    wave_lds_sync();
+    return k;
+}
+
+__global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
+    __shared__ uint32_t sHash[kWavesPerBlock][kHashCap];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t w = threadIdx.x >> 6;
+    uint32_t* hk = sHash[w];
+    const uint64_t gwave = uint64_t(blockIdx.x) * kWavesPerBlock + w;
+    uint32_t* temp = a.temp + gwave * uint64_t(KZ_MAX_OPS) * a.temp_cap;
+    const int s = a.s;
+    unsigned long long acc[KZ_MAX_AGGREGATORS];
+#pragma unroll
+    for (int i = 0; i < KZ_MAX_AGGREGATORS; ++i) acc[i] = 0;
+    while (true) {
+        uint64_t first = 0;
+        if (lane == 0) first = atomicAdd(reinterpret_cast<unsigned long long*>(a.cursor), (unsigned long long)kBatch);
+        first = __shfl(first, 0) + a.begin;
+        if (first >= a.end) break;
+        const uint64_t last = first + kBatch < a.end ? first + kBatch : a.end;
+        for (uint64_t e = first; e < last; ++e) {
+            // rebuild the embedding (src/engine.cc:1008-1021)
+            uint32_t m[KZ_MAX_PATTERN];
+            uint64_t idx[KZ_MAX_PATTERN + 1];
+            idx[s] = e;
+            uint64_t cur = e;
+            for (int t = s; t >= 2; --t) {
+                m[t - 1] = a.lv[t].vertex[cur];
+                cur = a.lv[t].parent[cur];
+                idx[t - 1] = cur;
+            }
+            m[0] = a.lv[1].root_base + uint32_t(cur);
+            const uint32_t code = order_code(m, s);
+            const uint8_t li = a.types[code];
+            if (li == 0xFF) {
+                if (lane == 0) atomicOr(a.err, 1u);
+                continue;
+            }
+            const kz_step_list& L = a.lists[li];
+            // sorted distinct matched values for position classes
+            uint32_t z[KZ_MAX_PATTERN];
+            int q = 0;
+            for (int i = 0; i < s; ++i) {
+                bool dup = false;
+                for (int j = 0; j < q; ++j) dup |= z[j] == m[i];
+                if (!dup) z[q++] = m[i];
+            }
+            for (int i = 1; i < q; ++i)
+                for (int j = i; j > 0 && z[j - 1] > z[j]; --j) {
+                    const uint32_t t = z[j];
+                    z[j] = z[j - 1];
+                    z[j - 1] = t;
+                }
+            uint32_t rlen[KZ_MAX_OPS];
+            auto resolve = [&](const kz_set_ref& r) -> SetView {
+                SetView v;
+                if (r.kind == KZ_SET_NBR) {
+                    const uint32_t x = m[r.a];
+                    const uint64_t b = a.rowptr[x];
+                    v.p = a.col + b;
+                    v.n = uint32_t(a.rowptr[x + 1] - b);
+                } else if (r.a == s) {
+                    v.p = temp + uint64_t(r.b) * a.temp_cap;
+                    v.n = rlen[r.b];
+                } else {
+                    const DevLevel& Lv = a.lv[r.a];
+                    const uint64_t i = idx[r.a];
+                    v.p = Lv.slot_data[r.b] + (Lv.slot_off[r.b][i] - Lv.slot_base[r.b]);
+                    v.n = Lv.slot_len[r.b][i];
+                }
+                return v;
+            };
+            for (int j = 0; j < L.num_ops; ++j) {
+                const kz_op& op = L.ops[j];
+                rlen[j] = wave_set_op(op.op, resolve(op.lhs), resolve(op.rhs), temp + uint64_t(j) * a.temp_cap, hk);
+            }
+            uint32_t nch = 0;
+            uint64_t cbase = 0;
+            if (a.mode == 1) cbase = a.child_off[e - a.begin] - a.child_base;
+            for (int o = 0; o < L.num_outs; ++o) {
+                const kz_out& out = L.outs[o];
+                const SetView src = resolve(out.src);
+                uint32_t cnt = 0;
+                for (uint32_t base = 0; base < src.n; base += 64) {
+                    const uint32_t i = base + lane;
+                    bool pass = false;
+                    uint32_t v = 0;
+                    if (i < src.n) {
+                        v = src.p[i];
+                        pass = (out.pred >> pos_class(v, z, q)) & 1u;
+                    }
+                    const uint64_t msk = __ballot(pass);
+                    if (out.kind == KZ_OUT_SCATTER && a.mode == 1 && pass) {
+                        const uint64_t at = cbase + nch + cnt + __popcll(msk & lanemask_lt());
+                        a.child_parent[at] = uint32_t(e);
+                        a.child_vertex[at] = v;
+                    }
+                    cnt += __popcll(msk);
+                }
+                if (out.kind == KZ_OUT_COUNT) {
+                    if (a.mode == 0) {
+                        switch (out.aggregator) {
+                            case 0: acc[0] += cnt; break;
+                            case 1: acc[1] += cnt; break;
+                            case 2: acc[2] += cnt; break;
+                            case 3: acc[3] += cnt; break;
+                            case 4: acc[4] += cnt; break;
+                            case 5: acc[5] += cnt; break;
+                            case 6: acc[6] += cnt; break;
+                            default: acc[7] += cnt; break;
+                        }
+                    }
+                } else {
+                    nch += cnt;
+                }
+            }
+            if (a.mode == 0) {
+                if (lane == 0) {
+                    if (a.nchild) a.nchild[e - a.begin] = nch;
+                    for (int j = 0; j < KZ_MAX_OPS; ++j)
+                        if (a.cached_mask & (1u << j)) a.slot_len_out[j][e - a.begin] = j < L.num_ops ? rlen[j] : 0;
+                }
+            } else {
+                for (int j = 0; j < L.num_ops; ++j) {
+                    if (!(a.cached_mask & (1u << j))) continue;
+                    uint32_t* dst = a.slot_data_out[j] + (a.slot_off_out[j][e - a.begin] - a.slot_base_out[j]);
+                    const uint32_t* srcp = temp + uint64_t(j) * a.temp_cap;
+                    for (uint32_t i = lane; i < rlen[j]; i += 64) dst[i] = srcp[i];
+                }
+            }
+        }
+    }
+    if (lane == 0 && a.mode == 0) {
+        for (int i = 0; i < KZ_MAX_AGGREGATORS; ++i)
+            if (acc[i]) atomicAdd(&a.agg[i], acc[i]);
+    }
+}
+
+// Smallest e in [0, n] with prefix[e] >= target (prefix has n+1 entries).
+__global__ void chunk_bounds_kernel(const uint64_t* __restrict__ cost, uint64_t n, uint64_t budget, uint32_t nb,
+                                    uint64_t* __restrict__ bounds) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nb) return;
+    if (k == nb) {
+        bounds[k] = n;
+        return;
+    }
+    const uint64_t target = uint64_t(k) * budget;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (cost[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    bounds[k] = lo;
+}
+
+// cost[e] = 8 * child_off[e] + 4 * sum_j slot_off[j][e]  (all exclusive prefixes, n+1 entries)
+__global__ void chunk_cost_kernel(uint64_t n, const uint64_t* __restrict__ child_off, const uint64_t* __restrict__ s0,
+                                  const uint64_t* __restrict__ s1, const uint64_t* __restrict__ s2,
+                                  const uint64_t* __restrict__ s3, uint64_t* __restrict__ cost) {
+    const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e > n) return;
+    uint64_t c = 8 * (child_off ? child_off[e] : 0);
+    if (s0) c += 4 * s0[e];
+    if (s1) c += 4 * s1[e];
+    if (s2) c += 4 * s2[e];
+    if (s3) c += 4 * s3[e];
+    cost[e] = c;
+}
+
+struct CastU64 {
+    __host__ __device__ uint64_t operator()(uint32_t x) const { return x; }
+};
+
+// Exclusive scan of u32 counts into u64 offsets with n+1 entries.
+static int scan_u32(const uint32_t* in, uint64_t* out, uint64_t n, Scratch& tmp, hipStream_t s) {
+    hipcub::TransformInputIterator<uint64_t, CastU64, const uint32_t*> it(in, CastU64());
+    size_t bytes = 0;
+    KZ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, it, out, n + 1, s));
+    if (int e = scratch_reserve(tmp, bytes)) return e;
+    KZ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, bytes, it, out, n + 1, s));
+    return KZ_OK;
+}
+
+// ----------------------------------------------------------------- host --
+
+struct LevelScratch {
+    Scratch nchild, child_off, slot_len[KZ_MAX_OPS], slot_off[KZ_MAX_OPS], cost, bounds;
+    Scratch child_parent, child_vertex, slot_data[KZ_MAX_OPS];
+};
+
+struct Executor {
+    DeviceGraph& g;
+    const PlanDesc& p;
+    kz_step_list* d_lists = nullptr;
+    uint8_t* d_types = nullptr;
+    uint32_t* d_cursor = nullptr;
+    unsigned long long* d_agg = nullptr;
+    uint32_t* d_err = nullptr;
+    Scratch temp, scan_tmp;
+    LevelScratch ls[KZ_MAX_PATTERN + 1];
+    uint32_t cached_mask[KZ_MAX_PATTERN + 1] = {};
+    bool has_scatter[KZ_MAX_PATTERN + 1] = {};
+    uint32_t grid = 0;
+    uint32_t temp_cap = 0;
+    kz_run_stats* st = nullptr;
+    uint64_t peak = 0;
+    uint64_t in_use = 0;
+
+    Executor(DeviceGraph& g_, const PlanDesc& p_) : g(g_), p(p_) {}
+    ~Executor() {
+        hipFree(d_lists);
+        hipFree(d_types);
+        hipFree(d_cursor);
+        hipFree(d_agg);
+        hipFree(d_err);
+        auto rel = [](Scratch& s) {
+            if (s.ptr) hipFree(s.ptr);
+            s.ptr = nullptr;
+        };
+        rel(temp);
+        rel(scan_tmp);
+        for (auto& l : ls) {
+            rel(l.nchild);
+            rel(l.child_off);
+            rel(l.cost);
+            rel(l.bounds);
+            rel(l.child_parent);
+            rel(l.child_vertex);
+            for (int j = 0; j < KZ_MAX_OPS; ++j) {
+                rel(l.slot_len[j]);
+                rel(l.slot_off[j]);
+                rel(l.slot_data[j]);
+            }
+        }
+    }
+
+    int setup() {
+        const int K = int(p.pattern_size);
+        // which slots of size s are read by larger sizes (must be cached)
+        for (uint32_t li = 0; li < p.num_step_lists; ++li) {
+            const kz_step_list& L = p.lists[li];
+            auto mark = [&](const kz_set_ref& r, int at) {
+                if (r.kind == KZ_SET_SLOT && r.a < at) cached_mask[r.a] |= 1u << r.b;
+            };
+            int at = -1;
+            for (int s = 1; s < K && at < 0; ++s)
+                for (int c = 0; c < 256; ++c)
+                    if (p.type_table[s][c] == li) {
+                        at = s;
+                        break;
+                    }
+            if (at < 0) continue;
+            for (int j = 0; j < L.num_ops; ++j) {
+                mark(L.ops[j].lhs, at);
+                mark(L.ops[j].rhs, at);
+            }
+            for (int o = 0; o < L.num_outs; ++o) {
+                mark(L.outs[o].src, at);
+                if (L.outs[o].kind == KZ_OUT_SCATTER) has_scatter[at] = true;
+            }
+        }
+        KZ_HIP(hipMalloc(&d_lists, sizeof(kz_step_list) * std::max<uint32_t>(1, p.num_step_lists)));
+        KZ_HIP(hipMalloc(&d_types, sizeof(p.type_table)));
+        KZ_HIP(hipMalloc(&d_cursor, 8));
+        KZ_HIP(hipMalloc(&d_agg, sizeof(unsigned long long) * KZ_MAX_AGGREGATORS));
+        KZ_HIP(hipMalloc(&d_err, 4));
+        hipStream_t s = rt().stream;
+        KZ_HIP(hipMemcpyAsync(d_lists, p.lists, sizeof(kz_step_list) * p.num_step_lists, hipMemcpyHostToDevice, s));
+        KZ_HIP(hipMemcpyAsync(d_types, p.type_table, sizeof(p.type_table), hipMemcpyHostToDevice, s));
+        KZ_HIP(hipMemsetAsync(d_agg, 0, sizeof(unsigned long long) * KZ_MAX_AGGREGATORS, s));
+        KZ_HIP(hipMemsetAsync(d_err, 0, 4, s));
+        grid = uint32_t(rt().num_cus) * 4;
+        temp_cap = std::max<uint32_t>(g.max_degree, 1);
+        if (int e = scratch_reserve(temp, size_t(grid) * kWavesPerBlock * KZ_MAX_OPS * temp_cap * 4)) return e;
+        return KZ_OK;
+    }
+
+    int launch(LevelArgs& la) {
+        hipStream_t s = rt().stream;
+        KZ_HIP(hipMemsetAsync(d_cursor, 0, 8, s));
+        la.cursor = d_cursor;
+        la.agg = d_agg;
+        la.err = d_err;
+        la.lists = d_lists;
+        la.types = d_types + la.s * 256;
+        la.rowptr = g.rowptr;
+        la.col = g.col;
+        la.temp = static_cast<uint32_t*>(temp.ptr);
+        la.temp_cap = temp_cap;
+        hipLaunchKernelGGL(plan_level_kernel, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, la);
+        KZ_HIP(hipGetLastError());
+        if (st) st->launches++;
+        return KZ_OK;
+    }
+
+    // Extend every embedding of size s held in lv[s] (all ancestors in lv).
+    int run_level(int s, DevLevel* lv) {
+        const int K = int(p.pattern_size);
+        const uint64_t n = lv[s].n;
+        if (!n) return KZ_OK;
+        if (st) st->embeddings[s] += n;
+        hipStream_t stream = rt().stream;
+        LevelScratch& L = ls[s];
+        const bool deeper = (s + 1 < K) && has_scatter[s];
+        LevelArgs la;
+        std::memset(&la, 0, sizeof(la));
+        for (int t = 1; t <= s; ++t) la.lv[t] = lv[t];
+        la.s = s;
+        la.begin = 0;
+        la.end = n;
+        la.mode = 0;
+        const uint32_t cm = deeper ? cached_mask[s] : 0u;
+        la.cached_mask = cm;
+        if (deeper) {
+            // n + 1 entries: the scans read one element past the last embedding
+            if (int e = scratch_reserve(L.nchild, (n + 1) * 4)) return e;
+            la.nchild = static_cast<uint32_t*>(L.nchild.ptr);
+            for (int j = 0; j < KZ_MAX_OPS; ++j)
+                if (cm & (1u << j)) {
+                    if (int e = scratch_reserve(L.slot_len[j], (n + 1) * 4)) return e;
+                    la.slot_len_out[j] = static_cast<uint32_t*>(L.slot_len[j].ptr);
+                }
+        }
+        if (int e = launch(la)) return e;
+        if (!deeper) return KZ_OK;
+        // scans
+        if (int e = scratch_reserve(L.child_off, (n + 1) * 8)) return e;
+        uint64_t* child_off = static_cast<uint64_t*>(L.child_off.ptr);
+        if (int e = scan_u32(la.nchild, child_off, n, scan_tmp, stream)) return e;
+        uint64_t* slot_off[KZ_MAX_OPS] = {};
+        for (int j = 0; j < KZ_MAX_OPS; ++j)
+            if (cm & (1u << j)) {
+                if (int e = scratch_reserve(L.slot_off[j], (n + 1) * 8)) return e;
+                slot_off[j] = static_cast<uint64_t*>(L.slot_off[j].ptr);
+                if (int e = scan_u32(la.slot_len_out[j], slot_off[j], n, scan_tmp, stream)) return e;
+            }
+        if (int e = scratch_reserve(L.cost, (n + 1) * 8)) return e;
+        uint64_t* cost = static_cast<uint64_t*>(L.cost.ptr);
+        hipLaunchKernelGGL(chunk_cost_kernel, dim3((n + 1 + 255) / 256), dim3(256), 0, stream, n, child_off,
+                           slot_off[0], slot_off[1], slot_off[2], slot_off[3], cost);
+        uint64_t total = 0;
+        KZ_HIP(hipMemcpyAsync(&total, cost + n, 8, hipMemcpyDeviceToHost, stream));
+        KZ_HIP(hipStreamSynchronize(stream));
+        const uint64_t budget = std::max<uint64_t>(g_level_budget, 1 << 20);
+        uint32_t nb = uint32_t(std::max<uint64_t>(1, (total + budget - 1) / budget));
+        if (int e = scratch_reserve(L.bounds, (size_t(nb) + 1) * 8)) return e;
+        uint64_t* d_bounds = static_cast<uint64_t*>(L.bounds.ptr);
+        hipLaunchKernelGGL(chunk_bounds_kernel, dim3((nb + 1 + 255) / 256), dim3(256), 0, stream, cost, n, budget, nb,
+                           d_bounds);
+        std::vector<uint64_t> bounds(size_t(nb) + 1);
+        KZ_HIP(hipMemcpyAsync(bounds.data(), d_bounds, (size_t(nb) + 1) * 8, hipMemcpyDeviceToHost, stream));
+        KZ_HIP(hipStreamSynchronize(stream));
+        bounds[0] = 0;
+        bounds[nb] = n;
+        for (uint32_t c = 0; c < nb; ++c) {
+            const uint64_t p0 = bounds[c], p1 = bounds[c + 1];
+            if (p1 <= p0) continue;
+            // sizes of this chunk
+            uint64_t h[2 + 2 * KZ_MAX_OPS];
+            KZ_HIP(hipMemcpyAsync(&h[0], child_off + p0, 8, hipMemcpyDeviceToHost, stream));
+            KZ_HIP(hipMemcpyAsync(&h[1], child_off + p1, 8, hipMemcpyDeviceToHost, stream));
+            for (int j = 0; j < KZ_MAX_OPS; ++j)
+                if (slot_off[j]) {
+                    KZ_HIP(hipMemcpyAsync(&h[2 + 2 * j], slot_off[j] + p0, 8, hipMemcpyDeviceToHost, stream));
+                    KZ_HIP(hipMemcpyAsync(&h[3 + 2 * j], slot_off[j] + p1, 8, hipMemcpyDeviceToHost, stream));
+                }
+            KZ_HIP(hipStreamSynchronize(stream));
+            const uint64_t nchild = h[1] - h[0];
+            if (st) st->chunks++;
+            if (int e = scratch_reserve(L.child_parent, std::max<uint64_t>(1, nchild) * 4)) return e;
+            if (int e = scratch_reserve(L.child_vertex, std::max<uint64_t>(1, nchild) * 4)) return e;
+            LevelArgs lb = la;
+            lb.mode = 1;
+            lb.begin = p0;
+            lb.end = p1;
+            lb.nchild = nullptr;
+            lb.child_off = child_off + p0;
+            lb.child_base = h[0];
+            lb.child_parent = static_cast<uint32_t*>(L.child_parent.ptr);
+            lb.child_vertex = static_cast<uint32_t*>(L.child_vertex.ptr);
+            DevLevel cur = lv[s];
+            for (int j = 0; j < KZ_MAX_OPS; ++j) {
+                if (!slot_off[j]) continue;
+                const uint64_t len = h[3 + 2 * j] - h[2 + 2 * j];
+                if (int e = scratch_reserve(L.slot_data[j], std::max<uint64_t>(1, len) * 4)) return e;
+                lb.slot_off_out[j] = slot_off[j] + p0;
+                lb.slot_base_out[j] = h[2 + 2 * j];
+                lb.slot_data_out[j] = static_cast<uint32_t*>(L.slot_data[j].ptr);
+                cur.slot_off[j] = slot_off[j];
+                cur.slot_len[j] = la.slot_len_out[j];
+                cur.slot_data[j] = lb.slot_data_out[j];
+                cur.slot_base[j] = h[2 + 2 * j];
+            }
+            if (int e = launch(lb)) return e;
+            track();
+            DevLevel next[KZ_MAX_PATTERN + 1];
+            for (int t = 1; t <= s; ++t) next[t] = lv[t];
+            next[s] = cur;
+            next[s + 1] = DevLevel();
+            next[s + 1].n = nchild;
+            next[s + 1].parent = lb.child_parent;
+            next[s + 1].vertex = lb.child_vertex;
+            if (int e = run_level(s + 1, next)) return e;
+        }
+        return KZ_OK;
+    }
+
+    void track() {
+        uint64_t tot = temp.size + scan_tmp.size;
+        for (auto& l : ls) {
+            tot += l.nchild.size + l.child_off.size + l.cost.size + l.bounds.size + l.child_parent.size +
+                   l.child_vertex.size;
+            for (int j = 0; j < KZ_MAX_OPS; ++j) tot += l.slot_len[j].size + l.slot_off[j].size + l.slot_data[j].size;
+        }
+        peak = std::max(peak, tot);
+    }
+};
+
+int plan_execute(DeviceGraph& g, const PlanDesc& p, bool, uint64_t* raw_counts, kz_run_stats* st) {
+    if (int e = plan_validate(p)) return e;
+    if (g.n == 0) {
+        for (uint32_t i = 0; i < p.num_aggregators; ++i) raw_counts[i] = 0;
+        return KZ_OK;
+    }
+    Executor x(g, p);
+    x.st = st;
+    if (int e = x.setup()) return e;
+    hipStream_t s = rt().stream;
+    hipEvent_t ev0, ev1;
+    KZ_HIP(hipEventCreate(&ev0));
+    KZ_HIP(hipEventCreate(&ev1));
+    KZ_HIP(hipEventRecord(ev0, s));
+    DevLevel lv[KZ_MAX_PATTERN + 1];
+    lv[1].n = g.n;
+    lv[1].root_base = 0;
+    int e = x.run_level(1, lv);
+    KZ_HIP(hipEventRecord(ev1, s));
+    unsigned long long agg[KZ_MAX_AGGREGATORS];
+    uint32_t err = 0;
+    KZ_HIP(hipMemcpyAsync(agg, x.d_agg, sizeof(agg), hipMemcpyDeviceToHost, s));
+    KZ_HIP(hipMemcpyAsync(&err, x.d_err, 4, hipMemcpyDeviceToHost, s));
+    KZ_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    KZ_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    if (e) return e;
+    if (err) KZ_FAIL(KZ_ERR_UNSUPPORTED, "an embedding reached an order type the plan does not define");
+    for (uint32_t i = 0; i < p.num_aggregators && i < KZ_MAX_AGGREGATORS; ++i) raw_counts[i] = agg[i];
+    x.track();
+    if (st) {
+        st->device_ms = ms;
+        st->peak_bytes = x.peak;
+    }
+    return KZ_OK;
+}
+
+// ------------------------------------------------------------ plan utils --
+
+static uint32_t ipow(uint32_t b, int e) {
+    uint32_t r = 1;
+    while (e-- > 0) r *= b;
+    return r;
+}
+
+int plan_validate(const PlanDesc& p) {
+    const int K = int(p.pattern_size);
+    if (K < 2 || K > KZ_MAX_PATTERN) KZ_FAIL(KZ_ERR_INVALID, "pattern_size %d outside 2..%d", K, KZ_MAX_PATTERN);
+    if (p.num_aggregators > KZ_MAX_AGGREGATORS || p.num_step_lists > KZ_MAX_STEP_LISTS)
+        KZ_FAIL(KZ_ERR_INVALID, "too many aggregators or step lists");
+    for (int s = 1; s < K; ++s) {
+        const uint32_t ncode = ipow(uint32_t(s), s);
+        for (uint32_t c = 0; c < 256; ++c) {
+            const uint8_t li = p.type_table[s][c];
+            if (li == 0xFF) continue;
+            if (c >= ncode) KZ_FAIL(KZ_ERR_INVALID, "order code %u invalid for size %d", c, s);
+            if (li >= p.num_step_lists) KZ_FAIL(KZ_ERR_INVALID, "step list %u out of range", li);
+            const kz_step_list& L = p.lists[li];
+            if (L.num_ops > KZ_MAX_OPS || L.num_outs > KZ_MAX_OUTS) KZ_FAIL(KZ_ERR_INVALID, "step list too long");
+            auto ok_ref = [&](const kz_set_ref& r, int j_limit) {
+                if (r.kind == KZ_SET_NBR) return int(r.a) < s;
+                if (r.kind == KZ_SET_SLOT) {
+                    if (r.a < 1 || int(r.a) > s || r.b >= KZ_MAX_OPS) return false;
+                    if (int(r.a) == s) return int(r.b) < j_limit;
+                    return true;
+                }
+                return false;
+            };
+            for (int j = 0; j < L.num_ops; ++j) {
+                const kz_op& o = L.ops[j];
+                if (o.op != KZ_OP_INTERSECT && o.op != KZ_OP_SUBTRACT) KZ_FAIL(KZ_ERR_INVALID, "bad op");
+                if (!ok_ref(o.lhs, j) || !ok_ref(o.rhs, j)) KZ_FAIL(KZ_ERR_INVALID, "bad operand at size %d op %d", s, j);
+            }
+            for (int o = 0; o < L.num_outs; ++o) {
+                const kz_out& out = L.outs[o];
+                if (!ok_ref(out.src, L.num_ops)) KZ_FAIL(KZ_ERR_INVALID, "bad output source at size %d", s);
+                if (out.kind == KZ_OUT_COUNT && out.aggregator >= p.num_aggregators)
+                    KZ_FAIL(KZ_ERR_INVALID, "aggregator %u out of range", out.aggregator);
+                if (out.kind == KZ_OUT_SCATTER && s + 1 >= K)
+                    KZ_FAIL(KZ_ERR_INVALID, "scatter at the last size %d", s);
+                if (out.kind != KZ_OUT_COUNT && out.kind != KZ_OUT_SCATTER) KZ_FAIL(KZ_ERR_INVALID, "bad output kind");
+            }
+        }
+    }
+    return KZ_OK;
+}
+
+// kautomine k-clique schedule: size 1 scatters v > v0 from N(v0); size 2
+// caches N(v0) & N(v1) (four_clique.cc:88-115); every deeper size
+// intersects the parent's cached prefix with N(v_{s-1}) and scatters or
+// counts v > v_{s-1} (four_clique.cc:116-142, five_clique.cc:116-175).
+int plan_clique(int k, PlanDesc* out) {
+    if (k < 3 || k > KZ_MAX_PATTERN) KZ_FAIL(KZ_ERR_UNSUPPORTED, "clique size %d", k);
+    std::memset(out, 0, sizeof(*out));
+    std::memset(out->type_table, 0xFF, sizeof(out->type_table));
+    out->pattern_size = uint32_t(k);
+    out->num_aggregators = 1;
+    out->multiplicity[0] = 1;
+    for (int s = 1; s < k; ++s) {
+        kz_step_list& L = out->lists[s - 1];
+        if (s >= 2) {
+            L.num_ops = 1;
+            L.ops[0].op = KZ_OP_INTERSECT;
+            if (s == 2) {
+                L.ops[0].lhs = kz_set_ref{KZ_SET_NBR, 0, 0, 0};
+            } else {
+                L.ops[0].lhs = kz_set_ref{KZ_SET_SLOT, uint8_t(s - 1), 0, 0};
+            }
+            L.ops[0].rhs = kz_set_ref{KZ_SET_NBR, uint8_t(s - 1), 0, 0};
+        }
+        L.num_outs = 1;
+        kz_out& o = L.outs[0];
+        o.kind = (s + 1 == k) ? KZ_OUT_COUNT : KZ_OUT_SCATTER;
+        o.aggregator = 0;
+        o.child_active = 1;
+        o.src = s == 1 ? kz_set_ref{KZ_SET_NBR, 0, 0, 0} : kz_set_ref{KZ_SET_SLOT, uint8_t(s), 0, 0};
+        o.pred = 1u << (2 * s);  // v above every matched vertex
+        // increasing order type: dense ranks 0, 1, ..., s-1
+        uint32_t code = 0, pw = 1;
+        for (int j = 0; j < s; ++j) {
+            code += uint32_t(j) * pw;
+            pw *= uint32_t(s);
+        }
+        out->type_table[s][code] = uint8_t(s - 1);
+    }
+    out->num_step_lists = uint32_t(k - 1);
+    return KZ_OK;
+}
+
+}  // namespace kz
+
+using namespace kz;
+
+extern "C" {
+
+int kz_plan_clique(int k, kz_plan_desc* out) {
+    if (!out) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    return plan_clique(k, out);
+}
+
+int kz_plan_validate(const kz_plan_desc* plan) {
+    if (!plan) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    return plan_validate(*plan);
+}
+
+int kz_count_plan(kz_graph g, const kz_plan_desc* plan, uint64_t* raw_counts, uint32_t num_counts,
+                  kz_run_stats* stats) {
+    if (!g || !plan || !raw_counts) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    if (num_counts < plan->num_aggregators) KZ_FAIL(KZ_ERR_INVALID, "raw_counts holds fewer than num_aggregators");
+    if (int e = ensure_ready()) return e;
+    auto t0 = std::chrono::steady_clock::now();
+    kz_run_stats st;
+    std::memset(&st, 0, sizeof(st));
+    uint64_t tmp[KZ_MAX_AGGREGATORS] = {0};
+    if (int e = plan_execute(g->g, *plan, false, tmp, &st)) return e;
+    for (uint32_t i = 0; i < plan->num_aggregators; ++i) raw_counts[i] = tmp[i];
+    st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return KZ_OK;
+}
+
+int kz_set_level_budget(uint64_t bytes) {
+    if (bytes < (uint64_t(1) << 20)) KZ_FAIL(KZ_ERR_INVALID, "level budget below 1 MiB");
+    g_level_budget = bytes;
+    return KZ_OK;
+}
+
+}  // extern "C"
