@@ -79,8 +79,14 @@ def build(verbose: bool = True) -> Path:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
           *[str(o) for _, o in jobs], "-lgomp"])
     os.replace(tmp, out)
+    # the Khuzdul application API on top of it, and the reference's apps
+    # compiled unchanged against it where the reference tree is present
+    khz = PKG / "khuzdul"
+    subprocess.run(["make", "-s", "-B", "-C", str(khz)], check=True)
+    if Path("/root/reference/distributed_applications").is_dir():
+        subprocess.run(["make", "-s", "-j8", "-C", str(khz), "apps"], check=True)
     if verbose:
-        print(f"built {out}")
+        print(f"built {out} and {LIB_DIR / 'libkhuzdul_amd.so'}")
     return out
 
 

@@ -1,0 +1,29 @@
+// Graph handle seen by apps (include/graph.h of the reference).  On this
+// engine the whole CSR lives in HBM (libkz_hip.so kz_graph); the host keeps
+// only its shape.  HUB_VERTEX_DEGREE_TH is kept for source compatibility.
+#ifndef GRAPH_H
+#define GRAPH_H
+
+#include <stdint.h>
+
+#include "types.h"
+#include "vertex_set.h"
+
+#define HUB_VERTEX_DEGREE_TH (1024 * 1024 * 1024)
+
+struct Empty {
+    operator VertexId() const { return 0; }
+} __attribute__((packed));
+
+class HashBasedNumaAwareDistributedCSRGraph {
+   public:
+    VertexId num_vertices_ = 0;
+    EdgeId num_edges_ = 0;
+    VertexId max_degree_ = 0;
+    bool oriented_ = false;
+    inline VertexId get_num_vertices() { return num_vertices_; }
+    inline EdgeId get_num_edges() { return num_edges_; }
+    inline VertexId get_max_degree() { return max_degree_; }
+};
+
+#endif

@@ -1,0 +1,298 @@
+// libkhuzdul_amd.so: the Khuzdul application API implemented on the MI355X
+// engine (libkz_hip.so).  The generated apps of the reference
+// (distributed_applications/kautomine/*.cc, kgraphpi/*.cc) compile against
+// the headers in ../include and link against this library unchanged.
+//
+// Replaces, per reference file:
+//   src/distributed_application.cc:66-113  constructor: load the .dgraph,
+//                                           bring the CSR into HBM
+//   src/distributed_application.cc:259-356  run(): probe apply() into a plan
+//                                           and count on the GPU
+//   src/distributed_sys.cc                  single-process runtime (no MPI)
+//   src/vertex_set.cc                       set operations (evaluated by the
+//                                           probe; the GPU runs the real ones)
+//   src/timer.cc:10-14                      get_time()
+//   src/engine.cc:1809-1826                 LocalPerformanceMetric statics
+
+#include <sys/stat.h>
+#include <sys/time.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "distributed_application.h"
+#include "distributed_sys.h"
+#include "engine.h"
+#include "kz.h"
+#include "probe.h"
+#include "timer.h"
+#include "vertex_set.h"
+
+// ----------------------------------------------------------------- runtime --
+
+extern "C" {
+int MPI_Barrier(MPI_Comm) { return MPI_SUCCESS; }
+int MPI_Allreduce(const void* sendbuf, void* recvbuf, int count, MPI_Datatype type, MPI_Op, MPI_Comm) {
+    size_t w = 4;
+    if (type == MPI_CHAR) w = 1;
+    else if (type == MPI_LONG_LONG || type == MPI_UNSIGNED_LONG || type == MPI_UNSIGNED_LONG_LONG || type == MPI_DOUBLE)
+        w = 8;
+    if (sendbuf != recvbuf) std::memcpy(recvbuf, sendbuf, w * size_t(count));
+    return MPI_SUCCESS;
+}
+int MPI_Comm_rank(MPI_Comm, int* rank) {
+    *rank = 0;
+    return MPI_SUCCESS;
+}
+int MPI_Comm_size(MPI_Comm, int* size) {
+    *size = 1;
+    return MPI_SUCCESS;
+}
+}
+
+DistributedSys* DistributedSys::instance_ = nullptr;
+void DistributedSys::init_distributed_sys() {
+    if (!instance_) instance_ = new DistributedSys();
+}
+void DistributedSys::finalize_distributed_sys() {
+    delete instance_;
+    instance_ = nullptr;
+    kz_shutdown();
+}
+DistributedSys* DistributedSys::get_instance() {
+    if (!instance_) init_distributed_sys();
+    return instance_;
+}
+
+int SharedMemorySys::get_num_threads() { return 1; }
+
+double get_time() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return double(tv.tv_sec) + double(tv.tv_usec) * 1e-6;
+}
+
+bool file_exits(const std::string& path) {
+    struct stat st;
+    return stat(path.c_str(), &st) == 0;
+}
+long file_size(const char* path) {
+    struct stat st;
+    return stat(path, &st) == 0 ? long(st.st_size) : -1;
+}
+
+void print_num_comparision() {}
+
+// --------------------------------------------------------------- VertexSet --
+
+void VertexSet::intersect_with(VertexSet* op, VertexSet* result) {
+    kzk::probe_set_op(KZ_OP_INTERSECT, this, op, result);
+}
+void VertexSet::subtracted_by(VertexSet* op, VertexSet* result) { kzk::probe_set_op(KZ_OP_SUBTRACT, this, op, result); }
+VertexId VertexSet::intersect_and_count(VertexSet*) {
+    std::fprintf(stderr, "VertexSet::intersect_and_count is not part of the GPU plan format\n");
+    std::exit(-1);
+}
+void VertexSet::bounded_intersect_with(VertexSet*, VertexSet*, VertexId) {
+    std::fprintf(stderr, "VertexSet::bounded_intersect_with is not part of the GPU plan format\n");
+    std::exit(-1);
+}
+void VertexSet::bounded_subtracted_by(VertexSet*, VertexSet*, VertexId) {
+    std::fprintf(stderr, "VertexSet::bounded_subtracted_by is not part of the GPU plan format\n");
+    std::exit(-1);
+}
+
+// ------------------------------------------------------------------ metrics --
+
+namespace Khuzdul {
+
+std::mutex LocalPerformanceMetric::mutex;
+double LocalPerformanceMetric::load_graph_time = 0;
+double LocalPerformanceMetric::cpu_utilization = 0;
+double LocalPerformanceMetric::gpu_time_ms = 0;
+double LocalPerformanceMetric::probe_time_ms = 0;
+unsigned long long LocalPerformanceMetric::num_runs = 0;
+unsigned long long LocalPerformanceMetric::num_launches = 0;
+unsigned long long LocalPerformanceMetric::num_extensions = 0;
+std::string LocalPerformanceMetric::last_method;
+
+void LocalPerformanceMetric::init_local_metrics() {
+    gpu_time_ms = probe_time_ms = 0;
+    num_runs = num_launches = num_extensions = 0;
+}
+
+void LocalPerformanceMetric::print_metrics(int runs) {
+    if (runs <= 0) runs = 1;
+    std::printf("\n**************** MI355X engine metrics ****************\n");
+    std::printf("    graph load time:          %.3f (s)\n", load_graph_time);
+    std::printf("    runs:                     %llu\n", num_runs);
+    std::printf("    GPU counting time / run:  %.3f (ms)\n", gpu_time_ms / runs);
+    std::printf("    plan probing time / run:  %.3f (ms)\n", probe_time_ms / runs);
+    std::printf("    kernel or graph launches: %llu\n", num_launches);
+    std::printf("    embeddings extended:      %llu\n", num_extensions);
+    std::printf("    last execution form:      %s\n", last_method.c_str());
+    std::printf("*******************************************************\n");
+}
+
+// ----------------------------------------------------- DistributedApplication --
+
+static void die_kz(int rc, const char* what) {
+    if (rc == KZ_OK) return;
+    std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, kz_last_error());
+    std::exit(-1);
+}
+
+DistributedApplication::DistributedApplication(int num_threads, std::string graph_path, int max_pattern_size,
+                                               ApplyFunction apply_fun, bool load_vertex_labels,
+                                               bool finer_grained_multithreading, long long memory_size_per_level,
+                                               VertexId cache_degree_threshold, double relative_cache_size) {
+    (void)finer_grained_multithreading;
+    (void)cache_degree_threshold;
+    (void)relative_cache_size;
+    assert(max_pattern_size >= 3);  // distributed_application.cc:71
+    if (max_pattern_size > MAX_PATTERN_SIZE) {
+        std::fprintf(stderr, "pattern size %d exceeds MAX_PATTERN_SIZE %d\n", max_pattern_size, MAX_PATTERN_SIZE);
+        std::exit(-1);
+    }
+    if (load_vertex_labels) {
+        std::fprintf(stderr, "vertex labels are not supported by this engine (labelled FSM is out of scope)\n");
+        std::exit(-1);
+    }
+    max_pattern_size_ = max_pattern_size;
+    max_depth_ = max_pattern_size - 2;
+    apply_fun_ = apply_fun;
+    node_id_ = 0;
+    num_nodes_ = 1;
+    num_sockets_ = 1;
+    // the thread count keeps the reference's meaning for Aggregator slots;
+    // counting itself runs on the GPU
+    num_threads_ = num_threads > 0 ? std::min(num_threads, MAX_NUM_THREADS) : 1;
+    memory_size_per_level_ = memory_size_per_level;
+    // KZ_PROBE_ONLY=1: extract and dump plans (KZ_PLAN_DUMP) without a GPU;
+    // every count is then reported as 0.
+    probe_only_ = std::getenv("KZ_PROBE_ONLY") != nullptr;
+    device_graph_ = nullptr;
+    if (probe_only_) {
+        std::printf("KZ_PROBE_ONLY: plans are extracted and dumped, nothing is counted\n");
+        return;
+    }
+    const char* dev = std::getenv("KZ_DEVICE");
+    die_kz(kz_init(dev ? std::atoi(dev) : 0), "kz_init");
+    if (memory_size_per_level > 0) die_kz(kz_set_level_budget(uint64_t(memory_size_per_level)), "kz_set_level_budget");
+
+    std::printf("Loading graph %s...\n", graph_path.c_str());
+    double t = -get_time();
+    kz_csr csr;
+    std::memset(&csr, 0, sizeof(csr));
+    die_kz(kz_dgraph_load(graph_path.c_str(), &csr), "kz_dgraph_load");
+    std::printf("    Number of vertices: %u\n", csr.num_vertices);
+    std::printf("    Number of edges: %lu\n", (unsigned long)csr.num_entries);
+    std::printf("    Max degree: %u\n", csr.max_degree);
+    graph_.num_vertices_ = csr.num_vertices;
+    graph_.num_edges_ = csr.num_entries;
+    graph_.max_degree_ = csr.max_degree;
+    graph_.oriented_ = csr.oriented != 0;
+    kz_graph g = nullptr;
+    die_kz(kz_graph_upload(&csr, &g), "kz_graph_upload");
+    kz_csr_release(&csr);
+    device_graph_ = g;
+    t += get_time();
+    LocalPerformanceMetric::load_graph_time = t;
+    std::printf("It takes %.3f seconds to load the graph into HBM.\n", t);
+}
+
+DistributedApplication::~DistributedApplication() {
+    if (device_graph_) kz_graph_release(static_cast<kz_graph>(device_graph_));
+    device_graph_ = nullptr;
+}
+
+void DistributedApplication::register_aggregator(void* a, AddFn add) {
+    aggregators_.push_back(a);
+    aggregator_add_.push_back(add);
+}
+
+void DistributedApplication::unregister_aggregator(void* a) {
+    for (size_t i = 0; i < aggregators_.size(); ++i)
+        if (aggregators_[i] == a) {
+            aggregators_.erase(aggregators_.begin() + long(i));
+            aggregator_add_.erase(aggregator_add_.begin() + long(i));
+            return;
+        }
+}
+
+static bool same_plan(const kz_plan_desc& a, const kz_plan_desc& b) {
+    if (a.pattern_size != b.pattern_size || a.num_step_lists != b.num_step_lists) return false;
+    if (std::memcmp(a.type_table, b.type_table, sizeof(a.type_table)) != 0) return false;
+    for (uint32_t i = 0; i < a.num_step_lists; ++i) {
+        const kz_step_list& x = a.lists[i];
+        const kz_step_list& y = b.lists[i];
+        if (x.num_ops != y.num_ops || x.num_outs != y.num_outs) return false;
+        if (std::memcmp(x.ops, y.ops, sizeof(kz_op) * x.num_ops) != 0) return false;
+        for (int o = 0; o < x.num_outs; ++o) {
+            const kz_out& p = x.outs[o];
+            const kz_out& q = y.outs[o];
+            if (p.kind != q.kind || p.pred != q.pred || std::memcmp(&p.src, &q.src, sizeof(p.src)) != 0) return false;
+            if (p.kind == KZ_OUT_COUNT && p.aggregator != q.aggregator) return false;
+        }
+    }
+    return true;
+}
+
+void DistributedApplication::run(bool disable_output, bool restrict_starting_vertex_label,
+                                 LabelId starting_vertex_label) {
+    (void)starting_vertex_label;
+    if (restrict_starting_vertex_label) {
+        std::fprintf(stderr, "label-restricted runs are not supported by this engine\n");
+        std::exit(-1);
+    }
+    kz_plan_desc plan;
+    std::string err;
+    double t = -get_time();
+    if (kzk::extract_plan(apply_fun_, max_pattern_size_, aggregators_, &plan, &err) != 0) {
+        std::fprintf(stderr, "cannot turn apply() into a GPU plan: %s\n", err.c_str());
+        std::exit(-1);
+    }
+    t += get_time();
+    LocalPerformanceMetric::probe_time_ms += t * 1e3;
+    die_kz(kz_plan_validate(&plan), "kz_plan_validate");
+    if (const char* dump = std::getenv("KZ_PLAN_DUMP")) {
+        // appends the raw kz_plan_desc of this run (one record per run())
+        if (FILE* f = std::fopen(dump, "ab")) {
+            std::fwrite(&plan, sizeof(plan), 1, f);
+            std::fclose(f);
+        }
+    }
+    if (probe_only_) return;
+
+    kz_graph g = static_cast<kz_graph>(device_graph_);
+    uint64_t counts[KZ_MAX_AGGREGATORS] = {0};
+    kz_run_stats st;
+    std::memset(&st, 0, sizeof(st));
+    const char* method = std::getenv("KZ_METHOD");
+    const bool force_plain = method && std::strcmp(method, "plain") == 0;
+    kz_plan_desc clique;
+    bool is_clique = plan.num_aggregators == 1 && max_pattern_size_ <= 5 &&
+                     kz_plan_clique(max_pattern_size_, &clique) == KZ_OK && same_plan(plan, clique);
+    if (is_clique && !force_plain) {
+        uint64_t c = 0;
+        die_kz(kz_count_cliques(g, max_pattern_size_, KZ_METHOD_PERSISTENT, &c, &st), "kz_count_cliques");
+        counts[0] = c;
+        LocalPerformanceMetric::last_method = "persistent clique kernels (one hipGraph launch)";
+    } else {
+        die_kz(kz_count_plan(g, &plan, counts, KZ_MAX_AGGREGATORS, &st), "kz_count_plan");
+        LocalPerformanceMetric::last_method = "plain level-by-level plan executor";
+    }
+    for (size_t i = 0; i < aggregators_.size() && i < KZ_MAX_AGGREGATORS; ++i)
+        aggregator_add_[i](aggregators_[i], counts[i]);
+    LocalPerformanceMetric::gpu_time_ms += st.device_ms;
+    LocalPerformanceMetric::num_launches += st.launches;
+    for (int s = 0; s <= KZ_MAX_PATTERN; ++s) LocalPerformanceMetric::num_extensions += st.embeddings[s];
+    LocalPerformanceMetric::num_runs += 1;
+    if (!disable_output)
+        std::printf("node: 0, GPU runtime: %.3f (ms), %s, %u step lists\n", st.device_ms,
+                    LocalPerformanceMetric::last_method.c_str(), plan.num_step_lists);
+}
+
+}  // namespace Khuzdul

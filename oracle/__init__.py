@@ -197,3 +197,21 @@ def py_cliques(rowptr, col, k: int) -> int:
 
 def reference_binary(app: str) -> Path:
     return REF_DIR / app
+
+
+def run_plan(plan, rowptr, col, threads: int = 0) -> list[int]:
+    """Raw aggregator sums of a kz_plan_desc (ctypes struct or bytes)."""
+    import ctypes as _C
+
+    rp = np.ascontiguousarray(rowptr, dtype=np.uint64)
+    cl = _u32(col) if len(col) else np.zeros(1, np.uint32)
+    raw = np.zeros(8, np.uint64)
+    buf = bytes(plan) if not isinstance(plan, (bytes, bytearray)) else bytes(plan)
+    cbuf = _C.create_string_buffer(buf, len(buf))
+    L = lib()
+    L.kzo_run_plan.argtypes = [_C.c_void_p, _C.POINTER(_C.c_uint64), _C.POINTER(_C.c_uint32), _C.c_uint32,
+                               _C.c_int, _C.POINTER(_C.c_uint64)]
+    L.kzo_run_plan.restype = _C.c_int
+    L.kzo_run_plan(_C.cast(cbuf, _C.c_void_p), _p64(rp), _p32(cl), len(rp) - 1, threads, _p64(raw))
+    n = int.from_bytes(buf[4:8], "little")
+    return [int(x) for x in raw[:n]]

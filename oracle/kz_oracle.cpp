@@ -23,6 +23,8 @@
 
 #include <omp.h>
 
+#include "kz.h"  // plan IR layout only
+
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -428,6 +430,126 @@ int kzo_run_app(int app, const uint64_t* rowptr, const uint32_t* col, uint32_t n
 }
 
 // ------------------------------------------------------------ brute force --
+
+// ------------------------------------------------------- plan interpreter --
+// Depth-first CPU interpreter of the plan IR (include/kz.h), restating the
+// engine semantics a plan encodes: every vertex is a root; an embedding of
+// size s runs the step list of its order code; set operations follow
+// vertex_set.cc (kzo_intersect / kzo_subtract above); COUNT adds the number
+// of candidates whose position-class bit is set, SCATTER recurses on them.
+
+namespace {
+
+struct PlanRun {
+    const kz_plan_desc* p;
+    G g;
+    uint64_t acc[KZ_MAX_AGGREGATORS];
+    // slot storage per size and op index for the current DFS path
+    std::vector<VertexId> slot[KZ_MAX_PATTERN + 1][KZ_MAX_OPS];
+    uint32_t slot_len[KZ_MAX_PATTERN + 1][KZ_MAX_OPS];
+    uint32_t maxd;
+
+    void init(uint32_t md) {
+        maxd = md;
+        for (auto& row : slot)
+            for (auto& v : row) v.assign(md + 1, 0);
+        std::memset(acc, 0, sizeof(acc));
+    }
+
+    static uint32_t code_of(const VertexId* m, int s) {
+        uint32_t code = 0, pw = 1;
+        for (int j = 0; j < s; ++j) {
+            uint32_t r = 0;
+            for (int i = 0; i < s; ++i) {
+                if (!(m[i] < m[j])) continue;
+                bool first = true;
+                for (int k = 0; k < i; ++k)
+                    if (m[k] == m[i]) first = false;
+                r += first;
+            }
+            code += r * pw;
+            pw *= uint32_t(s);
+        }
+        return code;
+    }
+
+    void set_of(const kz_set_ref& r, const VertexId* m, const VertexId** ptr, uint32_t* n) {
+        if (r.kind == KZ_SET_NBR) {
+            *ptr = g.nbr(m[r.a]);
+            *n = g.deg(m[r.a]);
+        } else {
+            *ptr = slot[r.a][r.b].data();
+            *n = slot_len[r.a][r.b];
+        }
+    }
+
+    void extend(VertexId* m, int s) {
+        if (s < 1 || s >= KZ_MAX_PATTERN || s >= int(p->pattern_size)) return;
+        const uint8_t li = p->type_table[s][code_of(m, s)];
+        if (li == 0xFF) return;
+        const kz_step_list& L = p->lists[li];
+        for (int j = 0; j < L.num_ops; ++j) {
+            const VertexId *a, *b;
+            uint32_t na, nb;
+            set_of(L.ops[j].lhs, m, &a, &na);
+            set_of(L.ops[j].rhs, m, &b, &nb);
+            slot_len[s][j] = L.ops[j].op == KZ_OP_INTERSECT ? kzo_intersect(a, na, b, nb, slot[s][j].data())
+                                                            : kzo_subtract(a, na, b, nb, slot[s][j].data());
+        }
+        VertexId z[KZ_MAX_PATTERN];
+        int q = 0;
+        for (int i = 0; i < s; ++i) z[q++] = m[i];
+        std::sort(z, z + q);
+        q = int(std::unique(z, z + q) - z);
+        for (int o = 0; o < L.num_outs; ++o) {
+            const kz_out& out = L.outs[o];
+            const VertexId* src;
+            uint32_t n;
+            set_of(out.src, m, &src, &n);
+            // copy: a child may overwrite slots of this size further down
+            std::vector<VertexId> cand(src, src + n);
+            for (VertexId v : cand) {
+                const uint32_t cls = 2 * uint32_t(std::lower_bound(z, z + q, v) - z) +
+                                     (std::binary_search(z, z + q, v) ? 1u : 0u);
+                if (!((out.pred >> cls) & 1u)) continue;
+                if (out.kind == KZ_OUT_COUNT) {
+                    acc[out.aggregator] += 1;
+                } else {
+                    m[s] = v;
+                    extend(m, s + 1);
+                }
+            }
+        }
+    }
+};
+
+}  // namespace
+
+// raw[a] = uint64 sum of aggregator a over all roots.
+int kzo_run_plan(const kz_plan_desc* plan, const uint64_t* rowptr, const uint32_t* col, uint32_t n, int threads,
+                 uint64_t* raw) {
+    uint32_t maxd = 0;
+    for (uint32_t v = 0; v < n; ++v) maxd = std::max(maxd, uint32_t(rowptr[v + 1] - rowptr[v]));
+    uint64_t tot[KZ_MAX_AGGREGATORS] = {0};
+    int T = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel num_threads(T)
+    {
+        PlanRun r;
+        r.p = plan;
+        r.g = G{rowptr, col, n};
+        r.init(maxd);
+#pragma omp for schedule(dynamic, 64)
+        for (uint64_t v = 0; v < n; ++v) {
+            VertexId m[KZ_MAX_PATTERN + 1];
+            m[0] = VertexId(v);
+            r.extend(m, 1);
+        }
+#pragma omp critical
+        for (int a = 0; a < KZ_MAX_AGGREGATORS; ++a) tot[a] += r.acc[a];
+    }
+    for (uint32_t a = 0; a < plan->num_aggregators && a < KZ_MAX_AGGREGATORS; ++a) raw[a] = tot[a];
+    return 0;
+}
 
 // Exhaustive k-subset enumeration (k = 3..5) with an adjacency matrix, for
 // n <= 256.  Counts k-cliques; pins the app restatements by closed form
