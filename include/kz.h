@@ -179,6 +179,28 @@ int kz_count_cliques(kz_graph g, int k, int method, uint64_t* count, kz_run_stat
  * reference reports no bandwidth for this path. */
 int kz_clique_workload(kz_graph g, uint64_t* out6);
 
+/* ---- Set primitives, batched -------------------------------------------
+ * Many (A, B) list pairs in one launch, one wave64 per pair.  Lists are
+ * packed in `values` with offsets list_off[0..num_lists] (strictly
+ * increasing ids per list); pair i uses lists a_idx[i] and b_idx[i].
+ *   KZ_SETOP_INTERSECT   VertexSet::intersect_with (src/vertex_set.cc:20-49)
+ *   KZ_SETOP_SUBTRACT    VertexSet::subtracted_by  (src/vertex_set.cc:51-86)
+ *     result i is written sorted at out_values + out_off[i]; the slot must
+ *     hold min(|A|,|B|) (intersect) or |A| (subtract) ids; out_len[i] gets
+ *     the result length;
+ *   KZ_SETOP_INTERSECT_COUNT  VertexSet::intersect_and_count
+ *     (src/vertex_set.cc:88-111): out_len[i] = |A & B|;
+ *   KZ_SETOP_TRIM_LOWER_BOUND trim_v_list_lower_bound (include/utilities.h:
+ *     76-86): out_len[i] = index of the first id of A above bounds[i].
+ * Host buffers in and out; b_idx is ignored by TRIM, bounds by the others. */
+#define KZ_SETOP_INTERSECT 1
+#define KZ_SETOP_SUBTRACT 2
+#define KZ_SETOP_INTERSECT_COUNT 3
+#define KZ_SETOP_TRIM_LOWER_BOUND 4
+int kz_set_op_batch(int op, const uint32_t* values, const uint64_t* list_off, uint32_t num_lists,
+                    const uint32_t* a_idx, const uint32_t* b_idx, const uint32_t* bounds, uint32_t num_pairs,
+                    uint32_t* out_values, const uint64_t* out_off, uint32_t* out_len);
+
 /* ---- Plan IR: a generated apply() schedule in data form ----------------
  * Replaces the ApplyFunction callback (include/engine.h:589) with a table the
  * GPU executes.  An embedding of size s (matched vertices m_0..m_{s-1}) is

@@ -149,6 +149,40 @@ def set_level_budget(nbytes: int) -> None:
     check(lib().kz_set_level_budget(int(nbytes)))
 
 
+def set_op_batch(op: str, lists, pairs, bounds=None):
+    """Batched VertexSet operations on the GPU (kz_set_op_batch).
+
+    op: "intersect" | "subtract" | "intersect_count" | "trim_lower_bound".
+    lists: sequence of strictly increasing id sequences; pairs: sequence of
+    (a, b) list indices (b ignored by trim).  Returns a list of numpy arrays
+    (intersect/subtract) or a numpy array of integers."""
+    code = {"intersect": _ffi.KZ_SETOP_INTERSECT, "subtract": _ffi.KZ_SETOP_SUBTRACT,
+            "intersect_count": _ffi.KZ_SETOP_INTERSECT_COUNT,
+            "trim_lower_bound": _ffi.KZ_SETOP_TRIM_LOWER_BOUND}[op]
+    arrs = [np.ascontiguousarray(x, dtype=np.uint32) for x in lists]
+    off = np.zeros(len(arrs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in arrs])
+    vals = np.concatenate(arrs) if arrs and off[-1] else np.zeros(1, np.uint32)
+    pa = np.ascontiguousarray([p[0] for p in pairs], dtype=np.uint32)
+    pb = np.ascontiguousarray([p[1] for p in pairs], dtype=np.uint32)
+    bd = np.ascontiguousarray(bounds if bounds is not None else np.zeros(len(pairs)), dtype=np.uint32)
+    sizes = []
+    for a, b in zip(pa, pb):
+        na, nb = len(arrs[a]), len(arrs[b])
+        sizes.append(min(na, nb) if code == _ffi.KZ_SETOP_INTERSECT else na)
+    oo = np.zeros(len(pairs) + 1, np.uint64)
+    oo[1:] = np.cumsum(sizes)
+    outv = np.zeros(max(1, int(oo[-1])), np.uint32)
+    outl = np.zeros(max(1, len(pairs)), np.uint32)
+    P32, P64 = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
+    check(lib().kz_set_op_batch(code, vals.ctypes.data_as(P32), off.ctypes.data_as(P64), len(arrs),
+                                pa.ctypes.data_as(P32), pb.ctypes.data_as(P32), bd.ctypes.data_as(P32), len(pairs),
+                                outv.ctypes.data_as(P32), oo.ctypes.data_as(P64), outl.ctypes.data_as(P32)))
+    if code in (_ffi.KZ_SETOP_INTERSECT, _ffi.KZ_SETOP_SUBTRACT):
+        return [outv[int(oo[i]):int(oo[i]) + int(outl[i])].copy() for i in range(len(pairs))]
+    return outl[:len(pairs)].copy()
+
+
 def clique_plan(k: int) -> _ffi.kz_plan_desc:
     p = _ffi.kz_plan_desc()
     check(lib().kz_plan_clique(k, C.byref(p)))

@@ -169,7 +169,15 @@ _SIGS = {
                                 C.POINTER(kz_run_stats)]),
     "kz_set_level_budget": (C.c_int, [C.c_uint64]),
     "kz_clique_workload": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kz_set_op_batch": (C.c_int, [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.c_uint32,
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32,
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
 }
+
+KZ_SETOP_INTERSECT = 1
+KZ_SETOP_SUBTRACT = 2
+KZ_SETOP_INTERSECT_COUNT = 3
+KZ_SETOP_TRIM_LOWER_BOUND = 4
 
 
 def lib() -> C.CDLL:

@@ -33,14 +33,15 @@
 #include "kz_device.h"
 #include "kz_kernels.h"
 #include "kz_plan.h"
+#include "kz_setops.h"
 
 namespace kz {
 
 uint64_t g_level_budget = uint64_t(4) << 30;  // memory_size_per_level default (4 GiB)
 
-constexpr uint32_t kHashCap = 2048;  // per-wave LDS hash slots
+
 constexpr int kWavesPerBlock = 4;
-constexpr uint32_t kBatch = 8;  // embeddings per cursor grab
+constexpr uint32_t kBatch = 32;  // embeddings per cursor grab (siblings stay together)
 
 struct DevLevel {
     uint64_t n = 0;
@@ -61,6 +62,7 @@ struct LevelArgs {
     const uint64_t* rowptr;
     const uint32_t* col;
     const kz_step_list* lists;
+    const uint8_t* fuse;   // per list: ops whose result only feeds COUNT outputs
     const uint8_t* types;  // type_table row for size s
     uint32_t* cursor;
     unsigned long long* agg;
@@ -81,104 +83,17 @@ struct LevelArgs {
     uint32_t temp_cap;
 };
 
-struct SetView {
-    const uint32_t* p;
-    uint32_t n;
-};
-
-__device__ __forceinline__ uint64_t lanemask_lt() {
-    const uint32_t lane = threadIdx.x & 63;
-    return lane ? (~0ull >> (64 - lane)) : 0ull;
-}
-
-// Position class of v against sorted distinct matched values z[0..q).
-__device__ __forceinline__ uint32_t pos_class(uint32_t v, const uint32_t* z, int q) {
-    uint32_t lt = 0, eq = 0;
-    for (int i = 0; i < q; ++i) {
-        lt += z[i] < v;
-        eq |= z[i] == v;
+__device__ __forceinline__ void add_agg(unsigned long long* acc, int agg, unsigned long long v) {
+    switch (agg) {
+        case 0: acc[0] += v; break;
+        case 1: acc[1] += v; break;
+        case 2: acc[2] += v; break;
+        case 3: acc[3] += v; break;
+        case 4: acc[4] += v; break;
+        case 5: acc[5] += v; break;
+        case 6: acc[6] += v; break;
+        default: acc[7] += v; break;
     }
-    return 2 * lt + eq;
-}
-
-// Wave-cooperative A op B.  Writes the ordered result to `out` (global) and
-// returns its length.  `hk` is this wave's LDS hash (kHashCap slots).
-__device__ uint32_t wave_set_op(int op, SetView A, SetView B, uint32_t* __restrict__ out, uint32_t* hk) {
-    const uint32_t lane = threadIdx.x & 63;
-    if (op == KZ_OP_INTERSECT && (A.n == 0 || B.n == 0)) return 0;
-    if (op == KZ_OP_SUBTRACT && A.n == 0) return 0;
-    // Choose the streamed list S and the probed list P.
-    SetView S, P;
-    if (op == KZ_OP_SUBTRACT) {
-        S = A;
-        P = B;
-    } else if (A.n <= B.n) {
-        S = B;
-        P = A;
-    } else {
-        S = A;
-        P = B;
-    }
-    // For an intersection with a very short P, stream P and search S instead.
-    bool search_mode;
-    if (op == KZ_OP_INTERSECT) {
-        if (uint64_t(S.n) > 32ull * P.n || P.n > kHashCap / 2) {
-            SetView t = S;  // stream the shorter, search the longer
-            S = P;
-            P = t;
-            search_mode = true;
-        } else {
-            search_mode = false;
-        }
-    } else {
-        search_mode = (P.n > kHashCap / 2) || (uint64_t(P.n) > 32ull * S.n);
-    }
-    uint32_t hmask = 0, hshift = 0;
-    if (!search_mode && P.n) {
-        uint32_t H = 64;
-        while (H < 2 * P.n) H <<= 1;
-        hmask = H - 1;
-        uint32_t bits = 0;
-        while ((1u << bits) < H) ++bits;
-        hshift = 32 - bits;
-        for (uint32_t i = lane; i < H; i += 64) hk[i] = kEmpty;
-        wave_lds_sync();
-        for (uint32_t i = lane; i < P.n; i += 64) {
-            const uint32_t x = P.p[i];
-            uint32_t h = (x * 2654435761u) >> hshift;
-            while (atomicCAS(&hk[h], kEmpty, x) != kEmpty) h = (h + 1) & hmask;
-        }
-        wave_lds_sync();
-    }
-    uint32_t k = 0;
-    for (uint32_t base = 0; base < S.n; base += 64) {
-        const uint32_t i = base + lane;
-        const bool valid = i < S.n;
-        const uint32_t x = valid ? S.p[i] : 0;
-        bool found = false;
-        if (valid) {
-            if (search_mode) {
-                found = bsearch_global(P.p, P.n, x);
-            } else if (P.n) {
-                uint32_t h = (x * 2654435761u) >> hshift;
-                while (true) {
-                    const uint32_t key = hk[h];
-                    if (key == x) {
-                        found = true;
-                        break;
-                    }
-                    if (key == kEmpty) break;
-                    h = (h + 1) & hmask;
-                }
-            }
-        }
-        const bool keep = valid && (op == KZ_OP_INTERSECT ? found : !found);
-        const uint64_t m = __ballot(keep);
-        if (keep) out[k + __popcll(m & lanemask_lt())] = x;
-        k += __popcll(m);
-    }
-    wave_lds_sync();
-    return k;
 }
 
 __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
@@ -189,6 +104,7 @@ __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
     const uint64_t gwave = uint64_t(blockIdx.x) * kWavesPerBlock + w;
     uint32_t* temp = a.temp + gwave * uint64_t(KZ_MAX_OPS) * a.temp_cap;
     const int s = a.s;
+    HashCache hc{nullptr, 0};
     unsigned long long acc[KZ_MAX_AGGREGATORS];
 #pragma unroll
     for (int i = 0; i < KZ_MAX_AGGREGATORS; ++i) acc[i] = 0;
@@ -217,6 +133,7 @@ __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
                 continue;
             }
             const kz_step_list& L = a.lists[li];
+            const uint32_t fuse = a.fuse[li];
             // sorted distinct matched values for position classes
             uint32_t z[KZ_MAX_PATTERN];
             int q = 0;
@@ -231,7 +148,7 @@ __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
                     z[j] = z[j - 1];
                     z[j - 1] = t;
                 }
-            uint32_t rlen[KZ_MAX_OPS];
+            uint32_t rlen[KZ_MAX_OPS] = {0, 0, 0, 0};
             auto resolve = [&](const kz_set_ref& r) -> SetView {
                 SetView v;
                 if (r.kind == KZ_SET_NBR) {
@@ -250,15 +167,45 @@ __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
                 }
                 return v;
             };
+            // an operand is shared by siblings when it comes from an ancestor
+            auto shared = [&](const kz_set_ref& r) {
+                return (r.kind == KZ_SET_NBR && int(r.a) < s - 1) || (r.kind == KZ_SET_SLOT && int(r.a) < s);
+            };
             for (int j = 0; j < L.num_ops; ++j) {
                 const kz_op& op = L.ops[j];
-                rlen[j] = wave_set_op(op.op, resolve(op.lhs), resolve(op.rhs), temp + uint64_t(j) * a.temp_cap, hk);
+                if (fuse & (1u << j)) {
+                    // result only feeds COUNT outputs: count while streaming
+                    if (a.mode != 0) continue;
+                    unsigned long long c[KZ_MAX_OUTS] = {0, 0, 0, 0};
+                    wave_set_op(op.op, resolve(op.lhs), resolve(op.rhs), shared(op.lhs), shared(op.rhs), hk, hc,
+                                [&](uint32_t x, bool keep, uint64_t, uint32_t) {
+                                    const uint32_t cls = keep ? pos_class(x, z, q) : 0;
+                                    for (int o = 0; o < L.num_outs; ++o) {
+                                        const kz_out& out = L.outs[o];
+                                        if (out.src.kind != KZ_SET_SLOT || out.src.a != s || out.src.b != j) continue;
+                                        c[o] += __popcll(__ballot(keep && ((out.pred >> cls) & 1u)));
+                                    }
+                                });
+                    for (int o = 0; o < L.num_outs; ++o) {
+                        const kz_out& out = L.outs[o];
+                        if (out.src.kind == KZ_SET_SLOT && out.src.a == s && out.src.b == j)
+                            add_agg(acc, out.aggregator, c[o]);
+                    }
+                    continue;
+                }
+                uint32_t* outp = temp + uint64_t(j) * a.temp_cap;
+                rlen[j] = wave_set_op(op.op, resolve(op.lhs), resolve(op.rhs), shared(op.lhs), shared(op.rhs), hk, hc,
+                                      [&](uint32_t x, bool keep, uint64_t msk, uint32_t k) {
+                                          if (keep) outp[k + __popcll(msk & lanemask_lt())] = x;
+                                      });
             }
             uint32_t nch = 0;
             uint64_t cbase = 0;
             if (a.mode == 1) cbase = a.child_off[e - a.begin] - a.child_base;
             for (int o = 0; o < L.num_outs; ++o) {
                 const kz_out& out = L.outs[o];
+                if (out.src.kind == KZ_SET_SLOT && out.src.a == s && (fuse & (1u << out.src.b))) continue;
+                if (out.kind == KZ_OUT_COUNT && a.mode != 0) continue;
                 const SetView src = resolve(out.src);
                 uint32_t cnt = 0;
                 for (uint32_t base = 0; base < src.n; base += 64) {
@@ -277,22 +224,8 @@ __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
                     }
                     cnt += __popcll(msk);
                 }
-                if (out.kind == KZ_OUT_COUNT) {
-                    if (a.mode == 0) {
-                        switch (out.aggregator) {
-                            case 0: acc[0] += cnt; break;
-                            case 1: acc[1] += cnt; break;
-                            case 2: acc[2] += cnt; break;
-                            case 3: acc[3] += cnt; break;
-                            case 4: acc[4] += cnt; break;
-                            case 5: acc[5] += cnt; break;
-                            case 6: acc[6] += cnt; break;
-                            default: acc[7] += cnt; break;
-                        }
-                    }
-                } else {
-                    nch += cnt;
-                }
+                if (out.kind == KZ_OUT_COUNT) add_agg(acc, out.aggregator, cnt);
+                else nch += cnt;
             }
             if (a.mode == 0) {
                 if (lane == 0) {
@@ -374,6 +307,7 @@ struct Executor {
     DeviceGraph& g;
     const PlanDesc& p;
     kz_step_list* d_lists = nullptr;
+    uint8_t* d_fuse = nullptr;
     uint8_t* d_types = nullptr;
     uint32_t* d_cursor = nullptr;
     unsigned long long* d_agg = nullptr;
@@ -391,6 +325,7 @@ struct Executor {
     Executor(DeviceGraph& g_, const PlanDesc& p_) : g(g_), p(p_) {}
     ~Executor() {
         hipFree(d_lists);
+        hipFree(d_fuse);
         hipFree(d_types);
         hipFree(d_cursor);
         hipFree(d_agg);
@@ -418,6 +353,7 @@ struct Executor {
 
     int setup() {
         const int K = int(p.pattern_size);
+        std::vector<int> list_at(std::max<uint32_t>(1, p.num_step_lists), -1);
         // which slots of size s are read by larger sizes (must be cached)
         for (uint32_t li = 0; li < p.num_step_lists; ++li) {
             const kz_step_list& L = p.lists[li];
@@ -440,7 +376,30 @@ struct Executor {
                 mark(L.outs[o].src, at);
                 if (L.outs[o].kind == KZ_OUT_SCATTER) has_scatter[at] = true;
             }
+            list_at[li] = at;
         }
+        // ops whose result feeds only COUNT outputs of the same step list
+        // (not cached, not read by a later op) are counted while streaming
+        std::vector<uint8_t> fuse(std::max<uint32_t>(1, p.num_step_lists), 0);
+        for (uint32_t li = 0; li < p.num_step_lists; ++li) {
+            const int at = list_at[li];
+            if (at < 0) continue;
+            const kz_step_list& L = p.lists[li];
+            for (int j = 0; j < L.num_ops; ++j) {
+                if (cached_mask[at] & (1u << j)) continue;
+                auto is_j = [&](const kz_set_ref& r) { return r.kind == KZ_SET_SLOT && r.a == at && r.b == j; };
+                bool ok = true, any = false;
+                for (int k = j + 1; k < L.num_ops; ++k) ok &= !is_j(L.ops[k].lhs) && !is_j(L.ops[k].rhs);
+                for (int o = 0; o < L.num_outs; ++o)
+                    if (is_j(L.outs[o].src)) {
+                        any = true;
+                        ok &= L.outs[o].kind == KZ_OUT_COUNT;
+                    }
+                if (ok && any) fuse[li] |= uint8_t(1u << j);
+            }
+        }
+        KZ_HIP(hipMalloc(&d_fuse, fuse.size()));
+        KZ_HIP(hipMemcpy(d_fuse, fuse.data(), fuse.size(), hipMemcpyHostToDevice));
         KZ_HIP(hipMalloc(&d_lists, sizeof(kz_step_list) * std::max<uint32_t>(1, p.num_step_lists)));
         KZ_HIP(hipMalloc(&d_types, sizeof(p.type_table)));
         KZ_HIP(hipMalloc(&d_cursor, 8));
@@ -464,6 +423,7 @@ struct Executor {
         la.agg = d_agg;
         la.err = d_err;
         la.lists = d_lists;
+        la.fuse = d_fuse;
         la.types = d_types + la.s * 256;
         la.rowptr = g.rowptr;
         la.col = g.col;
