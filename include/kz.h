@@ -169,6 +169,33 @@ typedef struct kz_run_stats {
  * forward out-degree <= 8192 (true after degree orientation). */
 int kz_count_cliques(kz_graph g, int k, int method, uint64_t* count, kz_run_stats* stats);
 
+/* Degree buckets of the persistent form: root tasks with |N+(u)| <= bound[0]
+ * run one per wave (bound[0] <= 64); up to bound[1..3] one per workgroup
+ * with bitmap rows in LDS (bound[3] <= 1024); larger ones keep rows in HBM.
+ * block[b] = threads per workgroup of bucket b (256 or 512; bucket 0: 256).
+ * No reference counterpart: the reference balances roots with 64-embedding
+ * chunks per thread (src/engine.cc:985-989). */
+typedef struct kz_bucket_cfg {
+    uint32_t bound[4];
+    uint32_t block[5];
+} kz_bucket_cfg;
+int kz_clique_get_buckets(kz_graph g, int k, kz_bucket_cfg* out);
+int kz_clique_set_buckets(kz_graph g, int k, const kz_bucket_cfg* cfg);
+/* Autotuner: times every candidate configuration (reps runs each, after one
+ * warm-up) on this graph and k, keeps the fastest for later calls and
+ * returns it with its average device time. */
+int kz_clique_tune(kz_graph g, int k, uint32_t reps, kz_bucket_cfg* chosen, double* best_ms);
+
+/* Batched form: many (graph, k) queries in one captured hipGraph launch.
+ * The batch keeps the graphs' handles, which must outlive it.  Query
+ * (g, j) counts ks[j]-cliques of graphs[g]; kz_batch_run writes
+ * counts[g * num_ks + j]. */
+typedef struct kz_batch_s* kz_batch;
+int kz_batch_create(const kz_graph* graphs, uint32_t num_graphs, const int* ks, uint32_t num_ks, kz_batch* out);
+int kz_batch_run(kz_batch b, uint64_t* counts, kz_run_stats* stats);
+int kz_batch_tune(kz_batch b, uint32_t reps, kz_bucket_cfg* chosen, double* best_ms);
+int kz_batch_release(kz_batch b);
+
 /* Exact terms of the merge-model byte count of the clique schedules on the
  * forward (v > u) graph, for throughput reporting:
  *   out[0] = m              out[1] = T = sum_{(u,v)} c(u,v)

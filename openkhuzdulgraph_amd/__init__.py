@@ -22,7 +22,7 @@ from . import _ffi
 from ._ffi import KzError, KzLibraryMissing, check, lib
 
 __all__ = [
-    "CSRGraph", "DeviceGraph", "KzError", "KzLibraryMissing", "init", "shutdown", "device_info",
+    "Batch", "CSRGraph", "DeviceGraph", "KzError", "KzLibraryMissing", "init", "shutdown", "device_info",
     "gen_erdos_renyi", "gen_rmat", "gen_chung_lu", "from_edges", "load_dgraph", "set_level_budget",
     "clique_plan",
 ]
@@ -183,6 +183,43 @@ def set_op_batch(op: str, lists, pairs, bounds=None):
     return outl[:len(pairs)].copy()
 
 
+class Batch:
+    """Many (graph, k) clique queries in one hipGraph launch (kz_batch)."""
+
+    def __init__(self, graphs, ks):
+        self.graphs = list(graphs)  # keep the device graphs alive
+        self.ks = list(ks)
+        hs = (C.c_void_p * len(self.graphs))(*[g._h for g in self.graphs])
+        karr = (C.c_int * len(self.ks))(*self.ks)
+        self._h = C.c_void_p()
+        check(lib().kz_batch_create(hs, len(self.graphs), karr, len(self.ks), C.byref(self._h)))
+        self.last_stats: dict = {}
+
+    def run(self) -> np.ndarray:
+        out = (C.c_uint64 * (len(self.graphs) * len(self.ks)))()
+        st = _ffi.kz_run_stats()
+        check(lib().kz_batch_run(self._h, out, C.byref(st)))
+        self.last_stats = st.as_dict()
+        return np.array(out, dtype=np.uint64).reshape(len(self.graphs), len(self.ks))
+
+    def tune(self, reps: int = 2) -> dict:
+        cfg = _ffi.kz_bucket_cfg()
+        ms = C.c_double()
+        check(lib().kz_batch_tune(self._h, reps, C.byref(cfg), C.byref(ms)))
+        return {"cfg": cfg.as_dict(), "ms": ms.value}
+
+    def release(self) -> None:
+        if self._h:
+            check(lib().kz_batch_release(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 def clique_plan(k: int) -> _ffi.kz_plan_desc:
     p = _ffi.kz_plan_desc()
     check(lib().kz_plan_clique(k, C.byref(p)))
@@ -214,6 +251,26 @@ class DeviceGraph:
         check(lib().kz_count_cliques(self._h, k, _METHODS[method], C.byref(out), C.byref(st)))
         self.last_stats = st.as_dict()
         return int(out.value)
+
+    def tune_buckets(self, k: int, reps: int = 2) -> dict:
+        """Autotune the degree-bucket thresholds of the persistent form."""
+        cfg = _ffi.kz_bucket_cfg()
+        ms = C.c_double()
+        check(lib().kz_clique_tune(self._h, k, reps, C.byref(cfg), C.byref(ms)))
+        return {"cfg": cfg.as_dict(), "ms": ms.value}
+
+    def get_buckets(self, k: int) -> dict:
+        cfg = _ffi.kz_bucket_cfg()
+        check(lib().kz_clique_get_buckets(self._h, k, C.byref(cfg)))
+        return cfg.as_dict()
+
+    def set_buckets(self, k: int, bound, block) -> None:
+        cfg = _ffi.kz_bucket_cfg()
+        for i, x in enumerate(bound):
+            cfg.bound[i] = x
+        for i, x in enumerate(block):
+            cfg.block[i] = x
+        check(lib().kz_clique_set_buckets(self._h, k, C.byref(cfg)))
 
     def clique_workload(self) -> dict:
         """Merge-model byte terms (kz_clique_workload) and B3/B4 in bytes."""

@@ -100,6 +100,13 @@ class kz_run_stats(C.Structure):
         }
 
 
+class kz_bucket_cfg(C.Structure):
+    _fields_ = [("bound", C.c_uint32 * 4), ("block", C.c_uint32 * 5)]
+
+    def as_dict(self) -> dict:
+        return {"bound": list(self.bound), "block": list(self.block)}
+
+
 class kz_set_ref(C.Structure):
     _fields_ = [("kind", C.c_uint8), ("a", C.c_uint8), ("b", C.c_uint8), ("reserved", C.c_uint8)]
 
@@ -169,6 +176,14 @@ _SIGS = {
                                 C.POINTER(kz_run_stats)]),
     "kz_set_level_budget": (C.c_int, [C.c_uint64]),
     "kz_clique_workload": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kz_clique_get_buckets": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(kz_bucket_cfg)]),
+    "kz_clique_set_buckets": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(kz_bucket_cfg)]),
+    "kz_clique_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.POINTER(kz_bucket_cfg), C.POINTER(C.c_double)]),
+    "kz_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_int), C.c_uint32,
+                                  C.POINTER(C.c_void_p)]),
+    "kz_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(kz_run_stats)]),
+    "kz_batch_tune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(kz_bucket_cfg), C.POINTER(C.c_double)]),
+    "kz_batch_release": (C.c_int, [C.c_void_p]),
     "kz_set_op_batch": (C.c_int, [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.c_uint32,
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32,
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),

@@ -1,13 +1,16 @@
-// C-ABI entry points for pattern counting on a resident graph.
+// C-ABI entry points for pattern counting on resident graphs.
 //
 // kz_count_cliques replaces DistributedApplication::run
 // (src/distributed_application.cc:259-356) + Aggregator<EdgeId>::evaluate
 // (include/engine.h:628-645) for the clique apps: it returns the same uint64
 // that the app prints after the multiplicity division (1 for cliques).
+// kz_batch_* run many (graph, k) queries in one hipGraph launch; the
+// reference runs one app and pattern per process.
 
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <memory>
 
 #include "kz_device.h"
 #include "kz_kernels.h"
@@ -72,9 +75,69 @@ int ensure_forward(DeviceGraph& g) {
     return KZ_OK;
 }
 
+static BucketCfg graph_buckets(const DeviceGraph& d, int k) {
+    if (!d.bucket_set[k]) return default_buckets();
+    BucketCfg c;
+    for (int b = 0; b < kNumBuckets; ++b) {
+        c.hi[b] = d.bucket_hi[k][b];
+        c.block[b] = d.bucket_block[k][b];
+    }
+    return c;
+}
+
+static void store_buckets(DeviceGraph& d, int k, const BucketCfg& c) {
+    for (int b = 0; b < kNumBuckets; ++b) {
+        d.bucket_hi[k][b] = c.hi[b];
+        d.bucket_block[k][b] = c.block[b];
+    }
+    d.bucket_set[k] = true;
+    d.clique_exec[k].reset();
+}
+
+static BucketCfg from_abi(const kz_bucket_cfg& a) {
+    BucketCfg c = default_buckets();
+    for (int b = 0; b < 4; ++b) c.hi[b] = a.bound[b];
+    for (int b = 0; b < 5; ++b) c.block[b] = a.block[b];
+    return c;
+}
+
+static void to_abi(const BucketCfg& c, kz_bucket_cfg* a) {
+    for (int b = 0; b < 4; ++b) a->bound[b] = c.hi[b];
+    for (int b = 0; b < 5; ++b) a->block[b] = c.block[b];
+}
+
+static std::shared_ptr<void> wrap_job(CliqueJob* j) {
+    return std::shared_ptr<void>(j, [](void* p) { delete static_cast<CliqueJob*>(p); });
+}
+
+// Prepared single-graph job for (graph, k), built on first use.
+static int graph_job(DeviceGraph& d, int k, CliqueJob** out) {
+    if (int e = ensure_forward(d)) return e;
+    auto& slot = d.clique_exec[k];
+    if (!slot) {
+        CliqueJob* j = new (std::nothrow) CliqueJob();
+        if (!j) KZ_FAIL(KZ_ERR_NOMEM, "clique job");
+        slot = wrap_job(j);
+        std::vector<CliqueQuery> q = {CliqueQuery{d.fwd_rowptr, d.fwd_col, &d.h_fdeg, k}};
+        if (int e = clique_job_prepare(*j, q, graph_buckets(d, k))) {
+            slot.reset();
+            return e;
+        }
+    }
+    *out = static_cast<CliqueJob*>(slot.get());
+    return KZ_OK;
+}
+
 }  // namespace kz
 
 using namespace kz;
+
+struct kz_batch_s {
+    std::vector<kz_graph> graphs;
+    std::vector<int> ks;
+    std::unique_ptr<CliqueJob> job;
+    BucketCfg cfg = default_buckets();
+};
 
 extern "C" {
 
@@ -84,26 +147,15 @@ int kz_count_cliques(kz_graph g, int k, int method, uint64_t* count, kz_run_stat
     if (int e = ensure_ready()) return e;
     auto t0 = std::chrono::steady_clock::now();
     DeviceGraph& d = g->g;
-    if (int e = ensure_forward(d)) return e;
     double dev_ms = 0;
     kz_run_stats st;
     std::memset(&st, 0, sizeof(st));
     if (method == KZ_METHOD_PERSISTENT) {
-        auto& slot = d.clique_exec[k];
-        if (!slot) slot = std::shared_ptr<void>(new CliquePlanExec(), [](void* p) { delete static_cast<CliquePlanExec*>(p); });
-        CliquePlanExec& x = *static_cast<CliquePlanExec*>(slot.get());
-        CliqueView v;
-        v.rowptr = d.fwd_rowptr;
-        v.col = d.fwd_col;
-        v.h_fdeg = d.h_fdeg;
-        v.owner = &d;
-        if (int e = clique_prepare(v, k, x)) {
-            slot.reset();
-            return e;
-        }
-        if (int e = clique_run(x, count, &dev_ms)) return e;
+        CliqueJob* j = nullptr;
+        if (int e = graph_job(d, k, &j)) return e;
+        if (int e = clique_job_run(*j, count, &dev_ms)) return e;
         st.launches = 1;
-        st.tasks = x.ntasks;
+        st.tasks = j->ntasks;
     } else if (method == KZ_METHOD_PLAIN) {
         PlanDesc pd;
         plan_clique(k, &pd);
@@ -118,6 +170,136 @@ int kz_count_cliques(kz_graph g, int k, int method, uint64_t* count, kz_run_stat
     st.device_ms = dev_ms;
     st.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (stats) *stats = st;
+    return KZ_OK;
+}
+
+int kz_clique_get_buckets(kz_graph g, int k, kz_bucket_cfg* out) {
+    if (!g || !out || k < 3 || k > 5) KZ_FAIL(KZ_ERR_INVALID, "bad argument");
+    to_abi(graph_buckets(g->g, k), out);
+    return KZ_OK;
+}
+
+int kz_clique_set_buckets(kz_graph g, int k, const kz_bucket_cfg* cfg) {
+    if (!g || !cfg || k < 3 || k > 5) KZ_FAIL(KZ_ERR_INVALID, "bad argument");
+    BucketCfg c = from_abi(*cfg);
+    if (int e = check_buckets(c)) return e;
+    store_buckets(g->g, k, c);
+    return KZ_OK;
+}
+
+int kz_clique_tune(kz_graph g, int k, uint32_t reps, kz_bucket_cfg* chosen, double* best_ms) {
+    if (!g || k < 3 || k > 5) KZ_FAIL(KZ_ERR_INVALID, "bad argument");
+    if (int e = ensure_ready()) return e;
+    DeviceGraph& d = g->g;
+    if (reps == 0) reps = 1;
+    double best = 1e300;
+    BucketCfg best_cfg = default_buckets();
+    for (const BucketCfg& c : tuning_candidates()) {
+        store_buckets(d, k, c);
+        CliqueJob* j = nullptr;
+        if (int e = graph_job(d, k, &j)) return e;
+        uint64_t cnt = 0;
+        double ms = 0, tot = 0;
+        if (int e = clique_job_run(*j, &cnt, &ms)) return e;  // warm-up
+        for (uint32_t r = 0; r < reps; ++r) {
+            if (int e = clique_job_run(*j, &cnt, &ms)) return e;
+            tot += ms;
+        }
+        if (tot / reps < best) {
+            best = tot / reps;
+            best_cfg = c;
+        }
+    }
+    store_buckets(d, k, best_cfg);
+    if (chosen) to_abi(best_cfg, chosen);
+    if (best_ms) *best_ms = best;
+    return KZ_OK;
+}
+
+int kz_batch_create(const kz_graph* graphs, uint32_t num_graphs, const int* ks, uint32_t num_ks, kz_batch* out) {
+    if (!graphs || !ks || !out || num_graphs == 0 || num_ks == 0) KZ_FAIL(KZ_ERR_INVALID, "bad argument");
+    if (int e = ensure_ready()) return e;
+    for (uint32_t j = 0; j < num_ks; ++j)
+        if (ks[j] < 3 || ks[j] > 5) KZ_FAIL(KZ_ERR_UNSUPPORTED, "clique size %d outside 3..5", ks[j]);
+    kz_batch b = new (std::nothrow) kz_batch_s();
+    if (!b) KZ_FAIL(KZ_ERR_NOMEM, "batch");
+    b->graphs.assign(graphs, graphs + num_graphs);
+    b->ks.assign(ks, ks + num_ks);
+    for (kz_graph g : b->graphs) {
+        if (!g) {
+            delete b;
+            KZ_FAIL(KZ_ERR_INVALID, "null graph in batch");
+        }
+        if (int e = ensure_forward(g->g)) {
+            delete b;
+            return e;
+        }
+    }
+    *out = b;
+    return KZ_OK;
+}
+
+static int batch_prepare(kz_batch b) {
+    if (b->job && b->job->ready) return KZ_OK;
+    b->job.reset(new CliqueJob());
+    std::vector<CliqueQuery> q;
+    for (kz_graph g : b->graphs)
+        for (int k : b->ks) q.push_back(CliqueQuery{g->g.fwd_rowptr, g->g.fwd_col, &g->g.h_fdeg, k});
+    if (int e = clique_job_prepare(*b->job, q, b->cfg)) {
+        b->job.reset();
+        return e;
+    }
+    return KZ_OK;
+}
+
+int kz_batch_run(kz_batch b, uint64_t* counts, kz_run_stats* stats) {
+    if (!b || !counts) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    if (int e = batch_prepare(b)) return e;
+    double ms = 0;
+    if (int e = clique_job_run(*b->job, counts, &ms)) return e;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->device_ms = ms;
+        stats->launches = 1;
+        stats->tasks = b->job->ntasks;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return KZ_OK;
+}
+
+int kz_batch_tune(kz_batch b, uint32_t reps, kz_bucket_cfg* chosen, double* best_ms) {
+    if (!b) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    if (reps == 0) reps = 1;
+    std::vector<uint64_t> counts(b->graphs.size() * b->ks.size());
+    double best = 1e300;
+    BucketCfg best_cfg = default_buckets();
+    for (const BucketCfg& c : tuning_candidates()) {
+        b->cfg = c;
+        b->job.reset();
+        if (int e = batch_prepare(b)) return e;
+        double ms = 0, tot = 0;
+        if (int e = clique_job_run(*b->job, counts.data(), &ms)) return e;
+        for (uint32_t r = 0; r < reps; ++r) {
+            if (int e = clique_job_run(*b->job, counts.data(), &ms)) return e;
+            tot += ms;
+        }
+        if (tot / reps < best) {
+            best = tot / reps;
+            best_cfg = c;
+        }
+    }
+    b->cfg = best_cfg;
+    b->job.reset();
+    if (chosen) to_abi(best_cfg, chosen);
+    if (best_ms) *best_ms = best;
+    return KZ_OK;
+}
+
+int kz_batch_release(kz_batch b) {
+    if (!b) return KZ_OK;
+    if (rt().ready) (void)hipStreamSynchronize(rt().stream);
+    delete b;
     return KZ_OK;
 }
 

@@ -52,8 +52,12 @@ struct DeviceGraph {
     uint32_t* fwd_col = nullptr;
     std::vector<uint32_t> h_fdeg;
     bool fwd_ready = false;
-    // prepared persistent clique executions, indexed by k (3..5)
+    // prepared persistent clique executions, indexed by k (3..5), and the
+    // degree-bucket configuration each uses (set or autotuned)
     std::shared_ptr<void> clique_exec[6];
+    bool bucket_set[6] = {};
+    uint32_t bucket_hi[6][5] = {};
+    uint32_t bucket_block[6][5] = {};
 };
 
 // Device scratch arena that grows on demand and is reused across calls.

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -143,24 +144,52 @@ struct CliqueView {
     DeviceGraph* owner = nullptr;
 };
 
-// Prepared persistent execution for one (graph, k).
-struct CliquePlanExec {
-    bool ready = false;
-    std::vector<uint64_t> off;  // bucket offsets into d_tasks
-    uint64_t ntasks = 0;
-    uint32_t max_task_deg = 0;
-    uint32_t slab_blocks = 0;
-    uint32_t* d_tasks = nullptr;
-    uint32_t* d_cursor = nullptr;
-    unsigned long long* d_result = nullptr;
-    unsigned long long* d_slab = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    ~CliquePlanExec();
+// Degree-bucket bounds on |N+(u)| (last bucket unbounded) and block sizes.
+struct BucketCfg {
+    uint32_t hi[kNumBuckets];
+    uint32_t block[kNumBuckets];
+};
+BucketCfg default_buckets();
+int check_buckets(const BucketCfg& c);
+std::vector<BucketCfg> tuning_candidates();
+
+// One counting query of the persistent form: k-cliques of one forward graph.
+struct CliqueQuery {
+    const uint64_t* rowptr;
+    const uint32_t* col;
+    const std::vector<uint32_t>* fdeg;  // host forward out-degrees
+    int k;
 };
 
-int clique_prepare(const CliqueView& gv, int k, CliquePlanExec& x);
-int clique_run(CliquePlanExec& x, uint64_t* count, double* device_ms);
+// Prepared persistent execution of a set of queries: task groups per
+// (k, bucket) and the captured hipGraph that runs them all.
+struct CliqueJob {
+    struct Group {
+        int k = 0;
+        int bucket = 0;
+        uint64_t first = 0, count = 0;
+        uint32_t dmax = 0;
+        int cursor = 0;
+    };
+    bool ready = false;
+    BucketCfg cfg{};
+    uint32_t nqueries = 0;
+    uint64_t ntasks = 0;
+    std::vector<Group> groups;
+    void* d_queries = nullptr;  // QueryRef[nqueries]
+    uint32_t* d_task_v = nullptr;
+    uint32_t* d_task_q = nullptr;  // null for a single query
+    uint32_t* d_cursor = nullptr;
+    unsigned long long* d_result = nullptr;
+    std::map<int, unsigned long long*> slabs;  // per k, HBM bitmap rows
+    std::map<int, uint32_t> slab_blocks;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~CliqueJob();
+};
+
+int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, const BucketCfg& cfg);
+int clique_job_run(CliqueJob& x, uint64_t* counts, double* device_ms);
 int clique_workload(const CliqueView& gv, uint64_t* out6);
 int ensure_forward(DeviceGraph& g);
 

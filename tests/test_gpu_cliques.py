@@ -61,3 +61,46 @@ def test_workload_terms_match_oracle(kzdev):
         assert got["B3"] == want["B3"]
         assert got["B4"] == want["B4"]
         assert got["triangles"] == want["triangles"]
+
+
+def test_batch_matches_single_runs(kzdev):
+    """Many (graph, k) queries in one hipGraph launch: each count equals the
+    oracle's, and equals the graph's own single-query run."""
+    kz = kzdev
+    import random
+
+    rng = random.Random(5)
+    hosts = []
+    for i in range(12):
+        n = rng.choice([300, 1000, 3000])
+        if i % 2:
+            g = kz.gen_erdos_renyi(n, n * rng.choice([4, 16, 40]), 100 + i)
+        else:
+            g = kz.gen_rmat(rng.choice([9, 10, 11]), rng.choice([4, 16, 32]), seed=100 + i)
+        hosts.append(g.oriented() if i % 3 else g)
+    devs = [g.to_device() for g in hosts]
+    b = kz.Batch(devs, [3, 4, 5])
+    for rep in range(3):
+        got = b.run()
+        for gi, g in enumerate(hosts):
+            for ki, (k, app) in enumerate(zip((3, 4, 5), (APP[3], APP[4], APP[5]))):
+                want = oracle.run_app(app, g.rowptr, g.col)[1][0]
+                assert int(got[gi, ki]) == want, (rep, gi, k)
+    for gi, d in enumerate(devs):
+        assert d.count_cliques(4, "persistent") == int(got[gi, 1])
+    b.release()
+
+
+def test_autotuner_keeps_counts_exact(kzdev):
+    kz = kzdev
+    g = kz.gen_rmat(13, 16, seed=2).oriented()
+    want = oracle.run_app(APP[4], g.rowptr, g.col)[1][0]
+    dg = g.to_device()
+    res = dg.tune_buckets(4, reps=1)
+    assert res["ms"] > 0 and res["cfg"]["bound"][0] in (32, 64)
+    assert dg.count_cliques(4, "persistent") == want
+    dg.set_buckets(4, [16, 128, 512, 1024], [256, 512, 512, 512, 512])
+    assert dg.count_cliques(4, "persistent") == want
+    with pytest.raises(kz.KzError):
+        dg.set_buckets(4, [128, 256, 512, 1024], [256, 256, 256, 512, 512])
+    dg.release()
