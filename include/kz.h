@@ -206,6 +206,19 @@ int kz_batch_release(kz_batch b);
  * reference reports no bandwidth for this path. */
 int kz_clique_workload(kz_graph g, uint64_t* out6);
 
+/* ---- Motif census -------------------------------------------------------
+ * Induced connected 3- and 4-vertex subgraph counts of a resident SYMMETRIC
+ * graph, counts[0..8) = {wedge (open 2-path), triangle, 3-path, 3-star,
+ * 4-cycle, tailed triangle, diamond, 4-clique}.  counts[0..2) are what the
+ * three_motif apps print (kautomine/three_motif.cc:59-139 with the
+ * multiplicities of :44; the kgraphpi variant counts non-induced wedges,
+ * kgraphpi/three_motif.cc:118-162); the reference has no 4-vertex census
+ * app.  The first call relabels the graph by (degree, id) rank on the GPU
+ * and keeps that copy with the graph.  Oriented graphs are rejected
+ * (KZ_ERR_INVALID). */
+#define KZ_CENSUS_PATTERNS 8
+int kz_motif_census(kz_graph g, uint64_t* counts, kz_run_stats* stats);
+
 /* ---- Set primitives, batched -------------------------------------------
  * Many (A, B) list pairs in one launch, one wave64 per pair.  Lists are
  * packed in `values` with offsets list_off[0..num_lists] (strictly

@@ -281,6 +281,15 @@ class DeviceGraph:
         return {"m": m, "triangles": T, "sum_c2": c2, "sum_tri_dw": sw, "sum_d2": dsq, "sum_dv": dv,
                 "B3": b3, "B4": b3 + 4 * (T + c2 + sw)}
 
+    def motif_census(self) -> list[int]:
+        """Induced 3- and 4-vertex census (kz_motif_census) of a symmetric
+        graph, in the order of _ffi.CENSUS_PATTERNS."""
+        out = (C.c_uint64 * _ffi.KZ_CENSUS_PATTERNS)()
+        st = _ffi.kz_run_stats()
+        check(lib().kz_motif_census(self._h, out, C.byref(st)))
+        self.last_stats = st.as_dict()
+        return [int(x) for x in out]
+
     def count_plan(self, plan: _ffi.kz_plan_desc) -> list[int]:
         n = max(1, int(plan.num_aggregators))
         out = (C.c_uint64 * _ffi.KZ_MAX_AGGREGATORS)()

@@ -26,6 +26,8 @@ KZ_MAX_AGGREGATORS = 8
 KZ_MAX_OPS = 4
 KZ_MAX_OUTS = 4
 KZ_MAX_STEP_LISTS = 128
+KZ_CENSUS_PATTERNS = 8
+CENSUS_PATTERNS = ("wedge", "triangle", "3-path", "3-star", "4-cycle", "tailed-triangle", "diamond", "4-clique")
 
 KZ_METHOD_PLAIN = 0
 KZ_METHOD_PERSISTENT = 1
@@ -176,6 +178,7 @@ _SIGS = {
                                 C.POINTER(kz_run_stats)]),
     "kz_set_level_budget": (C.c_int, [C.c_uint64]),
     "kz_clique_workload": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kz_motif_census": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(kz_run_stats)]),
     "kz_clique_get_buckets": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(kz_bucket_cfg)]),
     "kz_clique_set_buckets": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(kz_bucket_cfg)]),
     "kz_clique_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.POINTER(kz_bucket_cfg), C.POINTER(C.c_double)]),

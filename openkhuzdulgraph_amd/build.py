@@ -28,7 +28,8 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["kz_graph_host.cpp"]
-HIP_SOURCES = ["kz_runtime.hip", "kz_clique.hip", "kz_plan.hip", "kz_count.hip", "kz_setops.hip", "kz_workload.hip"]
+HIP_SOURCES = ["kz_runtime.hip", "kz_clique.hip", "kz_plan.hip", "kz_count.hip", "kz_setops.hip", "kz_workload.hip",
+               "kz_census.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

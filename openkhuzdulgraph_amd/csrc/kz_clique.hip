@@ -12,19 +12,25 @@
 //
 // MI355X design (no counterpart in the reference):
 //   * one task = one root vertex u of one query (graph, k).  N+(u) is staged
-//     in LDS together with an open-addressing hash (vertex id -> index);
+//     in LDS with an open-addressing hash (vertex id -> index);
 //   * the induced DAG on N+(u) is built as bitmap rows: row i holds bit j
-//     (j > i) iff N+(u)[j] is in N+(N+(u)[i]).  Each row is one coalesced
-//     stream over N+(N+(u)[i]) with hash probes, stopped at max N+(u); for
-//     skewed pairs (|N+(v)| much longer than the remaining candidates) the
-//     candidates are binary-searched in N+(v) instead;
+//     (j > i) iff N+(u)[j] is in N+(N+(u)[i]).  The lists N+(N+(u)[i]) of all
+//     rows are read as ONE stream of aligned 16-byte groups of 4 entries: an
+//     LDS prefix scan of the rows' group counts maps each lane's group index
+//     to (row, group), every lane keeps several independent 16-byte loads in
+//     flight and probes the hash with each entry (a wave walking one row at
+//     a time is bound by dependent load latency).  Rows whose list is much
+//     longer than their remaining candidates binary-search the candidates
+//     in the list instead;
 //   * cliques are counted with wave64 AND + popcount over the rows: k=4 sums
 //     popc(row_i & row_j) over DAG edges (i, j); k=5 descends one level more;
-//     k=3 counts row bits while they are found;
+//     k=3 counts the matches directly;
 //   * tasks are bucketed by |N+(u)| with runtime bounds (autotunable): the
 //     first bucket (<= 64) runs one task per wave with one u64 row per
 //     candidate, the middle buckets one task per workgroup with rows in LDS,
-//     the last keeps rows in a per-workgroup HBM slab;
+//     and roots above the last bound ("hubs") are split over many
+//     workgroups: a rows kernel builds chunks of rows into an HBM slab, then
+//     a count kernel takes one row per wave;
 //   * every (k, bucket) is a persistent kernel pulling tasks, largest first,
 //     from a device atomic cursor; all of them are captured in one hipGraph
 //     with parallel branches, replayed per run.  A batch of many graphs and
@@ -40,6 +46,7 @@
 
 #include "kz_device.h"
 #include "kz_kernels.h"
+#include "kz_stream.h"
 
 namespace kz {
 
@@ -48,15 +55,15 @@ struct QueryRef {
     const uint32_t* col;
 };
 
-__device__ __forceinline__ void flush_query(unsigned long long& acc, uint32_t q, unsigned long long* results) {
-    const unsigned long long s = wave_sum_u64(acc);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&results[q], s);
-    acc = 0;
-}
+// A hub root split over workgroups (bucket above the last bound).
+struct HubMeta {
+    uint32_t u, q, d, W;
+    uint64_t slab_off;  // in u64 words
+};
 
 // ------------------------------------------------------------ wave tasks --
-// One task per wave, |N+(u)| <= 64.  LDS per wave: L[64] ids, 128-slot hash,
-// 64 bitmap rows (one u64 each).
+// One task per wave, |N+(u)| <= 64.  LDS per wave: L[64], 128-slot hash,
+// 64 bitmap rows (one u64 each) and the row descriptors.
 template <int K>
 __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __restrict__ queries,
                                                           const uint32_t* __restrict__ task_v,
@@ -69,12 +76,16 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
     __shared__ uint32_t sKey[WPB][H];
     __shared__ uint16_t sVal[WPB][H];
     __shared__ unsigned long long sRow[WPB][64];
-    const int lane = threadIdx.x & 63;
+    __shared__ unsigned long long sRb[WPB][64];
+    __shared__ uint32_t sRp[WPB][64];
+    const uint32_t lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     uint32_t* L = sL[w];
     uint32_t* hk = sKey[w];
     uint16_t* hv = sVal[w];
     unsigned long long* rows = sRow[w];
+    unsigned long long* rb = sRb[w];
+    uint32_t* rp = sRp[w];
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
     const uint64_t* rowptr = nullptr;
@@ -92,41 +103,44 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
             col = queries[q].col;
         }
         const uint32_t u = task_v[t];
-        const uint64_t ub = rowptr[u];
-        const uint32_t d = uint32_t(rowptr[u + 1] - ub);
-        const uint32_t x = lane < d ? col[ub + lane] : 0xFFFFFFFFu;
+        const uint64_t ub = ldg(rowptr, u);
+        const uint32_t d = uint32_t(ldg(rowptr, u + 1) - ub);
+        const uint32_t x = lane < d ? ldg(col, ub + lane) : 0xFFFFFFFFu;
         L[lane] = x;
         hk[lane] = kEmpty;
         hk[lane + 64] = kEmpty;
         rows[lane] = 0;
+        // row descriptors: lane r owns row r
+        // (rows >= d - 1 are empty, so rp[d - 1] is the group total)
+        uint32_t groups = 0;
+        uint64_t desc = 0;
+        bool skewed = false;
+        if (lane + 1 < d) skewed = row_desc(rowptr, x, d - 1 - lane, &desc, &groups);
+        rb[lane] = desc;
+        uint32_t excl;
+        const uint32_t G = wave_exclusive_scan(groups, &excl);
+        rp[lane] = excl;
+        const uint64_t skmask = __ballot(skewed);
         wave_lds_sync();
         if (lane < d) hash_insert(hk, hv, H - 1, 25, x, uint16_t(lane));
         wave_lds_sync();
         const uint32_t lmax = L[d - 1];
-        for (uint32_t i = 0; i + 1 < d; ++i) {
-            const uint32_t v = L[i];
-            const uint64_t vb = rowptr[v];
-            const uint32_t dv = uint32_t(rowptr[v + 1] - vb);
-            const uint32_t rem = d - 1 - i;
-            if (dv > 16u * rem) {
-                const uint32_t j = i + 1 + lane;
-                if (j < d && bsearch_global(col + vb, dv, L[j])) {
-                    if (K == 3) acc += 1;
-                    else atomicOr(&rows[i], 1ull << j);
-                }
-            } else {
-                for (uint32_t base = 0; base < dv; base += 64) {
-                    const uint32_t p = base + lane;
-                    const uint32_t y = p < dv ? col[vb + p] : 0xFFFFFFFFu;
-                    if (y > v && y <= lmax) {
-                        const uint32_t j = hash_find(hk, hv, H - 1, 25, y);
-                        if (j != kNotFound) {
-                            if (K == 3) acc += 1;
-                            else atomicOr(&rows[i], 1ull << j);
-                        }
-                    }
-                    if (__ballot(y > lmax)) break;
-                }
+        stream_groups(col, rb, rp, d - 1, G, lane, 64, hk, hv, H - 1, 25, lmax,
+                      [&](uint32_t r, uint32_t j, uint64_t) {
+                          if (K == 3) acc += 1;
+                          else atomicOr(&rows[r], 1ull << j);
+                      });
+        // skewed rows: binary-search the remaining candidates in N+(v)
+        unsigned long long sm = skmask;
+        while (sm) {
+            const uint32_t i = uint32_t(__builtin_ctzll(sm));
+            sm &= sm - 1;
+            const uint64_t b = rb[i] & kStartMask;
+            const uint32_t n = uint32_t(ldg(rowptr, L[i] + 1) - b);
+            const uint32_t j = i + 1 + lane;
+            if (j < d && bsearch_global(col + b, n, L[j])) {
+                if (K == 3) acc += 1;
+                else atomicOr(&rows[i], 1ull << j);
             }
         }
         wave_lds_sync();
@@ -154,34 +168,126 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
 
+// Counts the k-cliques of one root's DAG rows i = first, first + stride, ...
+// (wave-cooperative: lanes take the set bits j of row i).  The bitmap is
+// word-major: word kk of row i is rows[kk * ws + i], so the lanes' rows j
+// sit in consecutive words (no LDS bank conflicts) and row i is a broadcast.
+template <int K>
+__device__ __forceinline__ void count_rows(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t first,
+                                           uint32_t stride, unsigned long long& acc) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wd = (d + 63) >> 6;
+    for (uint32_t i = first; i + 2 < d; i += stride) {
+        const unsigned long long* ri = rows + i;
+        const uint32_t k_lo = (i + 1) >> 6;
+        uint32_t total = 0;
+        for (uint32_t k = k_lo; k < wd; ++k) total += __popcll(ri[size_t(k) * ws]);
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t r = base + lane;
+            if (r >= total) break;
+            uint32_t k = k_lo, rem = r;
+            unsigned long long wv = ri[size_t(k) * ws];
+            for (uint32_t c = __popcll(wv); rem >= c; c = __popcll(wv)) {
+                rem -= c;
+                wv = ri[size_t(++k) * ws];
+            }
+            const uint32_t j = k * 64 + select_bit(wv, rem);
+            const unsigned long long* rj = rows + j;
+            const uint32_t kj = (j + 1) >> 6;
+            if (K == 4) {
+                for (uint32_t kk = kj; kk < wd; ++kk) acc += __popcll(ri[size_t(kk) * ws] & rj[size_t(kk) * ws]);
+            } else {
+                for (uint32_t kk = kj; kk < wd; ++kk) {
+                    unsigned long long m = ri[size_t(kk) * ws] & rj[size_t(kk) * ws];
+                    while (m) {
+                        const uint32_t l = kk * 64 + __builtin_ctzll(m);
+                        m &= m - 1;
+                        const unsigned long long* rl = rows + l;
+                        for (uint32_t k3 = (l + 1) >> 6; k3 < wd; ++k3)
+                            acc += __popcll(ri[size_t(k3) * ws] & rj[size_t(k3) * ws] & rl[size_t(k3) * ws]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Builds rows [r0, r1) of the DAG of root with list L (d ids, hash hk/hv)
+// into rows_out (word kk of row r at rows_out[kk * ws + r - r0], zeroed by
+// the caller),
+// streaming the neighbour lists of all rows as 16-byte groups.  rb holds
+// (r1 - r0) row descriptors and rp (r1 - r0 + 1) group prefixes, both LDS
+// scratch; tmp holds one u32 per wave.  Block-wide.
+template <int K>
+__device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                                 const uint32_t* L, uint32_t d, const uint32_t* hk, const uint16_t* hv,
+                                                 uint32_t hmask, uint32_t hshift, uint32_t r0, uint32_t r1,
+                                                 unsigned long long* rows_out, uint32_t ws, unsigned long long* rb,
+                                                 uint32_t* rp, uint32_t* tmp, unsigned long long& acc) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nwaves = blockDim.x >> 6;
+    const uint32_t nr = r1 - r0;
+    for (uint32_t r = threadIdx.x; r <= nr; r += blockDim.x) {
+        const uint32_t i = r0 + r;
+        uint32_t groups = 0;
+        uint64_t desc = 0;
+        if (r < nr && i + 1 < d) row_desc(rowptr, L[i], d - 1 - i, &desc, &groups);
+        if (r < nr) rb[r] = desc;
+        rp[r] = groups;
+    }
+    __syncthreads();
+    const uint32_t G = block_exclusive_scan(rp, nr + 1, tmp);
+    stream_groups(col, rb, rp, nr, G, threadIdx.x, blockDim.x, hk, hv, hmask, hshift, L[d - 1],
+                  [&](uint32_t r, uint32_t j, uint64_t) {
+                      if (K == 3) acc += 1;
+                      else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
+                  });
+    // skewed rows: one wave per row binary-searches the candidates in N+(v)
+    for (uint32_t r = wave; r < nr; r += nwaves) {
+        const uint64_t ds = rb[r];
+        if (!(ds & kSkewBit)) continue;
+        const uint32_t i = r0 + r;
+        const uint64_t b = ds & kStartMask;
+        const uint32_t n = uint32_t(ldg(rowptr, L[i] + 1) - b);
+        for (uint32_t j = i + 1 + lane; j < d; j += 64) {
+            if (bsearch_global(col + b, n, L[j])) {
+                if (K == 3) acc += 1;
+                else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // ----------------------------------------------------------- block tasks --
-// One task per workgroup.  Dynamic LDS: [rows Dmax*W u64 (k >= 4, rows in
-// LDS)][L Dmax u32][hash keys H u32][hash vals H u16].  With GLOBAL_ROWS the
-// rows live in an HBM slab of Dmax*W u64 per workgroup.
-template <int K, bool GLOBAL_ROWS>
+// One task per workgroup, rows in LDS.  Dynamic LDS: [rows Dmax*W u64 (k>=4)]
+// [row descriptors Dmax u64][L Dmax u32][group prefix Dmax+1 u32]
+// [hash keys H u32][hash vals H u16].
+template <int K>
 __global__ __launch_bounds__(512) void clique_block_kernel(const QueryRef* __restrict__ queries,
                                                            const uint32_t* __restrict__ task_v,
                                                            const uint32_t* __restrict__ task_q, uint32_t ntasks,
                                                            uint32_t* __restrict__ cursor,
                                                            unsigned long long* __restrict__ results, uint32_t Dmax,
-                                                           uint32_t W, uint32_t H, uint32_t hshift,
-                                                           unsigned long long* __restrict__ gslab) {
+                                                           uint32_t W, uint32_t H, uint32_t hshift) {
     extern __shared__ __align__(16) unsigned char smem[];
-    unsigned long long* rows = nullptr;
     unsigned char* p = smem;
-    if (GLOBAL_ROWS) {
-        rows = gslab + size_t(blockIdx.x) * Dmax * W;
-    } else if (K >= 4) {
+    unsigned long long* rows = nullptr;
+    if (K >= 4) {
         rows = reinterpret_cast<unsigned long long*>(p);
         p += size_t(Dmax) * W * 8;
     }
+    unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
+    p += size_t(Dmax) * 8;
     uint32_t* L = reinterpret_cast<uint32_t*>(p);
     p += size_t(Dmax) * 4;
+    uint32_t* rp = reinterpret_cast<uint32_t*>(p);
+    p += size_t(Dmax + 1) * 4;
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);
     p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
-    __shared__ uint32_t sTask;
-    const int lane = threadIdx.x & 63;
+    __shared__ uint32_t sTask, sTmp[8];
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
     const uint32_t hmask = H - 1;
@@ -202,86 +308,130 @@ __global__ __launch_bounds__(512) void clique_block_kernel(const QueryRef* __res
             col = queries[q].col;
         }
         const uint32_t u = task_v[t];
-        const uint64_t ub = rowptr[u];
-        const uint32_t d = uint32_t(rowptr[u + 1] - ub);
-        const uint32_t wd = (d + 63) >> 6;
-        for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) L[i] = col[ub + i];
+        const uint64_t ub = ldg(rowptr, u);
+        const uint32_t d = uint32_t(ldg(rowptr, u + 1) - ub);
+        for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) L[i] = ldg(col, ub + i);
         for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) hk[i] = kEmpty;
         if (K >= 4)
             for (uint32_t i = threadIdx.x; i < d * W; i += blockDim.x) rows[i] = 0;
-        if (GLOBAL_ROWS) __threadfence();
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) hash_insert(hk, hv, hmask, hshift, L[i], uint16_t(i));
         __syncthreads();
-        const uint32_t lmax = L[d - 1];
-        // rows: one wave per row i
-        for (uint32_t i = wave; i + 1 < d; i += nwaves) {
-            const uint32_t v = L[i];
-            const uint64_t vb = rowptr[v];
-            const uint32_t dv = uint32_t(rowptr[v + 1] - vb);
-            const uint32_t rem = d - 1 - i;
-            unsigned long long* ri = rows + size_t(i) * W;
-            if (dv > 16u * rem) {
-                for (uint32_t j = i + 1 + lane; j < d; j += 64) {
-                    if (bsearch_global(col + vb, dv, L[j])) {
-                        if (K == 3) acc += 1;
-                        else atomicOr(&ri[j >> 6], 1ull << (j & 63));
-                    }
-                }
-            } else {
-                for (uint32_t base = 0; base < dv; base += 64) {
-                    const uint32_t qq = base + lane;
-                    const uint32_t y = qq < dv ? col[vb + qq] : 0xFFFFFFFFu;
-                    if (y > v && y <= lmax) {
-                        const uint32_t j = hash_find(hk, hv, hmask, hshift, y);
-                        if (j != kNotFound) {
-                            if (K == 3) acc += 1;
-                            else atomicOr(&ri[j >> 6], 1ull << (j & 63));
-                        }
-                    }
-                    if (__ballot(y > lmax)) break;
-                }
-            }
-        }
-        if (GLOBAL_ROWS) __threadfence();
+        build_rows_block<K>(rowptr, col, L, d, hk, hv, hmask, hshift, 0, d, rows, d, rb, rp, sTmp, acc);
+        if (K >= 4) count_rows<K>(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
         __syncthreads();
+    }
+    if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
+}
+
+// ------------------------------------------------------------ hub tasks --
+// Hub roots are split: item c of hub h builds rows [c*RPC, (c+1)*RPC) in an
+// LDS buffer and stores them to the hub's HBM slab (k >= 4), or counts the
+// matches directly (k = 3).  Dynamic LDS: [row buffer RPC*Wmax u64 (k>=4)]
+// [row descriptors RPC u64][L Dmax u32][group prefix RPC+1 u32]
+// [hash keys H u32][hash vals H u16].
+template <int K>
+__global__ __launch_bounds__(512) void hub_rows_kernel(const QueryRef* __restrict__ queries,
+                                                       const HubMeta* __restrict__ hubs, uint32_t nhubs,
+                                                       const uint32_t* __restrict__ item_prefix, uint32_t nitems,
+                                                       uint32_t* __restrict__ cursor,
+                                                       unsigned long long* __restrict__ results,
+                                                       unsigned long long* __restrict__ slab, uint32_t Dmax,
+                                                       uint32_t Wmax, uint32_t RPC, uint32_t H) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned char* p = smem;
+    unsigned long long* rbuf = nullptr;
+    if (K >= 4) {
+        rbuf = reinterpret_cast<unsigned long long*>(p);
+        p += size_t(RPC) * Wmax * 8;
+    }
+    unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
+    p += size_t(RPC) * 8;
+    uint32_t* L = reinterpret_cast<uint32_t*>(p);
+    p += size_t(Dmax) * 4;
+    uint32_t* rp = reinterpret_cast<uint32_t*>(p);
+    p += size_t(RPC + 1) * 4;
+    uint32_t* hk = reinterpret_cast<uint32_t*>(p);
+    p += size_t(H) * 4;
+    uint16_t* hv = reinterpret_cast<uint16_t*>(p);
+    __shared__ uint32_t sItem, sTmp[8];
+    unsigned long long acc = 0;
+    uint32_t qcur = 0xFFFFFFFFu;
+    uint32_t hcur = 0xFFFFFFFFu;
+    const uint64_t* rowptr = nullptr;
+    const uint32_t* col = nullptr;
+    uint32_t hmask = 0, hshift = 0;
+    while (true) {
+        if (threadIdx.x == 0) sItem = atomicAdd(cursor, 1u);
+        __syncthreads();
+        const uint32_t it = sItem;
+        if (it >= nitems) break;
+        const uint32_t h = row_of(item_prefix, nhubs, it);
+        const HubMeta hm = hubs[h];
+        if (hm.q != qcur) {
+            if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
+            qcur = hm.q;
+            rowptr = queries[hm.q].rowptr;
+            col = queries[hm.q].col;
+        }
+        const uint32_t d = hm.d;
+        if (h != hcur) {  // stage this hub's list and hash once per run of its items
+            uint32_t Hh = 64;
+            while (Hh < 2 * d) Hh <<= 1;
+            uint32_t bits = 0;
+            while ((1u << bits) < Hh) ++bits;
+            hmask = Hh - 1;
+            hshift = 32 - bits;
+            const uint64_t ub = ldg(rowptr, hm.u);
+            for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) L[i] = ldg(col, ub + i);
+            for (uint32_t i = threadIdx.x; i < Hh; i += blockDim.x) hk[i] = kEmpty;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) hash_insert(hk, hv, hmask, hshift, L[i], uint16_t(i));
+            hcur = h;
+        }
+        const uint32_t r0 = (it - item_prefix[h]) * RPC;
+        const uint32_t r1 = r0 + RPC < d ? r0 + RPC : d;
+        if (K >= 4)
+            for (uint32_t i = threadIdx.x; i < (r1 - r0) * hm.W; i += blockDim.x) rbuf[i] = 0;
+        __syncthreads();
+        build_rows_block<K>(rowptr, col, L, d, hk, hv, hmask, hshift, r0, r1, rbuf, r1 - r0, rb, rp, sTmp, acc);
         if (K >= 4) {
-            // count: one wave per row i, lanes over the set bits j of row i
-            for (uint32_t i = wave; i + 2 < d; i += nwaves) {
-                const unsigned long long* ri = rows + size_t(i) * W;
-                const uint32_t k_lo = (i + 1) >> 6;
-                uint32_t total = 0;
-                for (uint32_t k = k_lo; k < wd; ++k) total += __popcll(ri[k]);
-                for (uint32_t base = 0; base < total; base += 64) {
-                    const uint32_t r = base + lane;
-                    if (r >= total) break;
-                    uint32_t k = k_lo, rem = r;
-                    unsigned long long wv = ri[k];
-                    for (uint32_t c = __popcll(wv); rem >= c; c = __popcll(wv)) {
-                        rem -= c;
-                        wv = ri[++k];
-                    }
-                    const uint32_t j = k * 64 + select_bit(wv, rem);
-                    const unsigned long long* rj = rows + size_t(j) * W;
-                    const uint32_t kj = (j + 1) >> 6;
-                    if (K == 4) {
-                        for (uint32_t kk = kj; kk < wd; ++kk) acc += __popcll(ri[kk] & rj[kk]);
-                    } else {
-                        for (uint32_t kk = kj; kk < wd; ++kk) {
-                            unsigned long long m = ri[kk] & rj[kk];
-                            while (m) {
-                                const uint32_t l = kk * 64 + __builtin_ctzll(m);
-                                m &= m - 1;
-                                const unsigned long long* rl = rows + size_t(l) * W;
-                                for (uint32_t k3 = (l + 1) >> 6; k3 < wd; ++k3)
-                                    acc += __popcll(ri[k3] & rj[k3] & rl[k3]);
-                            }
-                        }
-                    }
-                }
+            // the slab is word-major over all d rows of the hub
+            const uint32_t nr = r1 - r0;
+            unsigned long long* dst = slab + hm.slab_off + r0;
+            for (uint32_t i = threadIdx.x; i < nr * hm.W; i += blockDim.x) {
+                const uint32_t kk = i / nr;
+                dst[size_t(kk) * d + (i - kk * nr)] = rbuf[i];
             }
         }
         __syncthreads();
+    }
+    if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
+}
+
+// One wave per hub row: counts the cliques whose second vertex is that row.
+template <int K>
+__global__ __launch_bounds__(256) void hub_count_kernel(const HubMeta* __restrict__ hubs, uint32_t nhubs,
+                                                        const uint32_t* __restrict__ row_prefix, uint32_t nrows,
+                                                        uint32_t* __restrict__ cursor,
+                                                        unsigned long long* __restrict__ results,
+                                                        const unsigned long long* __restrict__ slab) {
+    const uint32_t lane = threadIdx.x & 63;
+    unsigned long long acc = 0;
+    uint32_t qcur = 0xFFFFFFFFu;
+    while (true) {
+        uint32_t r = 0;
+        if (lane == 0) r = atomicAdd(cursor, 1u);
+        r = __shfl(r, 0);
+        if (r >= nrows) break;
+        const uint32_t h = row_of(row_prefix, nhubs, r);
+        const HubMeta hm = hubs[h];
+        if (hm.q != qcur) {
+            if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
+            qcur = hm.q;
+        }
+        const uint32_t i = r - row_prefix[h];
+        count_rows<K>(slab + hm.slab_off, hm.d, hm.d, i, hm.d, acc);
     }
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
@@ -332,24 +482,35 @@ CliqueJob::~CliqueJob() {
     hipFree(d_cursor);
     hipFree(d_result);
     for (auto& s : slabs) hipFree(s.second);
+    for (auto& h : hub_meta) hipFree(h.second);
+    for (auto& h : hub_items) hipFree(h.second);
+    for (auto& h : hub_rows) hipFree(h.second);
 }
 
-static size_t lds_bytes(uint32_t Dmax, uint32_t W, uint32_t H, bool rows_in_lds) {
-    return (rows_in_lds ? size_t(Dmax) * W * 8 : 0) + size_t(Dmax) * 4 + size_t(H) * 4 + size_t(H) * 2 + 16;
+// Must match the carving at the top of clique_block_kernel / hub_rows_kernel.
+static size_t block_lds_bytes(int K, uint32_t Dmax, uint32_t W, uint32_t H) {
+    return (K >= 4 ? size_t(Dmax) * W * 8 : 0) + size_t(Dmax) * 8 + size_t(Dmax) * 4 + size_t(Dmax + 1) * 4 +
+           size_t(H) * 6 + 16;
+}
+
+static size_t hub_lds_bytes(int K, uint32_t Dmax, uint32_t Wmax, uint32_t RPC, uint32_t H) {
+    return (K >= 4 ? size_t(RPC) * Wmax * 8 : 0) + size_t(RPC) * 8 + size_t(RPC + 1) * 4 + size_t(Dmax) * 4 +
+           size_t(H) * 6 + 16;
 }
 
 static int set_lds_attrs() {
     static bool done = false;
     if (done) return KZ_OK;
-    // Dynamic LDS may use the block limit minus the kernel's static LDS; the
+    // Dynamic LDS may use the block limit minus the kernels' static LDS; the
     // attribute is advisory on this runtime, so a refusal is not fatal (the
     // launch itself reports a size it cannot honour).
     const int maxb = int(rt().lds_per_block) - kStaticLdsReserve;
-    const void* fns[] = {reinterpret_cast<const void*>(&clique_block_kernel<3, false>),
-                         reinterpret_cast<const void*>(&clique_block_kernel<4, false>),
-                         reinterpret_cast<const void*>(&clique_block_kernel<5, false>),
-                         reinterpret_cast<const void*>(&clique_block_kernel<4, true>),
-                         reinterpret_cast<const void*>(&clique_block_kernel<5, true>)};
+    const void* fns[] = {reinterpret_cast<const void*>(&clique_block_kernel<3>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<5>),
+                         reinterpret_cast<const void*>(&hub_rows_kernel<3>),
+                         reinterpret_cast<const void*>(&hub_rows_kernel<4>),
+                         reinterpret_cast<const void*>(&hub_rows_kernel<5>)};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, maxb) != hipSuccess)
             (void)hipGetLastError();
@@ -357,7 +518,16 @@ static int set_lds_attrs() {
     return KZ_OK;
 }
 
-// One (k, bucket) launch.
+static void hash_size(uint32_t Dmax, uint32_t* H, uint32_t* hshift) {
+    uint32_t h = 64;
+    while (h < 2 * Dmax) h <<= 1;
+    uint32_t bits = 0;
+    while ((1u << bits) < h) ++bits;
+    *H = h;
+    *hshift = 32 - bits;
+}
+
+// One (k, bucket) launch (two kernels for hubs).
 template <int K>
 static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s) {
     const Runtime& R = rt();
@@ -366,43 +536,99 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     const uint32_t* tv = x.d_task_v + gr.first;
     const uint32_t* tq = x.d_task_q ? x.d_task_q + gr.first : nullptr;
     uint32_t* cur = x.d_cursor + gr.cursor;
+    const size_t lds_limit = R.lds_per_block - kStaticLdsReserve;
     if (gr.bucket == 0) {
         const uint32_t grid = std::min<uint32_t>((nt + 3) / 4, uint32_t(R.num_cus) * 8);
-        hipLaunchKernelGGL(clique_wave_kernel<K>, dim3(grid), dim3(256), 0, s, qrefs, tv, tq, nt, cur,
-                           x.d_result);
+        hipLaunchKernelGGL(clique_wave_kernel<K>, dim3(grid), dim3(256), 0, s, qrefs, tv, tq, nt, cur, x.d_result);
         return KZ_OK;
     }
-    const uint32_t Dmax = gr.dmax;
-    const uint32_t W = (Dmax + 63) / 64;
-    uint32_t H = 64;
-    while (H < 2 * Dmax) H <<= 1;
-    uint32_t hbits = 0;
-    while ((1u << hbits) < H) ++hbits;
-    const uint32_t hshift = 32 - hbits;
-    // only k >= 4 builds bitmap rows; the last bucket keeps them in HBM
-    const bool global_rows = (gr.bucket == kNumBuckets - 1) && K >= 4;
-    const size_t lds = lds_bytes(Dmax, W, H, !global_rows && K >= 4);
+    if (gr.bucket < kNumBuckets - 1) {
+        const uint32_t Dmax = gr.dmax;
+        const uint32_t W = (Dmax + 63) / 64;
+        uint32_t H, hshift;
+        hash_size(Dmax, &H, &hshift);
+        const size_t lds = block_lds_bytes(K, Dmax, W, H);
+        const uint32_t block = x.cfg.block[gr.bucket];
+        if (lds > lds_limit)
+            KZ_FAIL(KZ_ERR_UNSUPPORTED, "bucket %d needs %zu bytes of LDS (limit %zu)", gr.bucket, lds, lds_limit);
+        uint32_t per_cu = std::max<uint32_t>(1, uint32_t(R.lds_per_block / (lds + kStaticLdsReserve)));
+        per_cu = std::min<uint32_t>(per_cu, 2048 / block);
+        const uint32_t grid = std::min<uint32_t>(nt, uint32_t(R.num_cus) * per_cu);
+        hipLaunchKernelGGL(clique_block_kernel<K>, dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt, cur,
+                           x.d_result, Dmax, W, H, hshift);
+        return KZ_OK;
+    }
+    // hubs: rows kernel, then (k >= 4) the count kernel on the same stream
+    const CliqueJob::HubPlan& hp = x.hubplan[K];
     const uint32_t block = x.cfg.block[gr.bucket];
-    if (lds + kStaticLdsReserve > R.lds_per_block)
-        KZ_FAIL(KZ_ERR_UNSUPPORTED, "bucket %d needs %zu bytes of LDS (limit %zu)", gr.bucket, lds, R.lds_per_block);
+    uint32_t H, hshift;
+    hash_size(hp.dmax, &H, &hshift);
+    (void)hshift;
+    const size_t lds = hub_lds_bytes(K, hp.dmax, hp.wmax, hp.rpc, H);
+    if (lds > lds_limit) KZ_FAIL(KZ_ERR_UNSUPPORTED, "hub rows need %zu bytes of LDS (limit %zu)", lds, lds_limit);
     uint32_t per_cu = std::max<uint32_t>(1, uint32_t(R.lds_per_block / (lds + kStaticLdsReserve)));
     per_cu = std::min<uint32_t>(per_cu, 2048 / block);
-    uint32_t grid = std::min<uint32_t>(nt, uint32_t(R.num_cus) * per_cu);
-    unsigned long long* slab = nullptr;
-    if (global_rows) {
-        auto it = x.slabs.find(K);
-        if (it == x.slabs.end()) KZ_FAIL(KZ_ERR_STATE, "missing row slab for k=%d", K);
-        slab = it->second;
-        grid = std::min<uint32_t>(grid, x.slab_blocks[K]);
+    const uint32_t grid = std::min<uint32_t>(hp.nitems, uint32_t(R.num_cus) * per_cu);
+    const HubMeta* meta = static_cast<const HubMeta*>(x.hub_meta[K]);
+    hipLaunchKernelGGL(hub_rows_kernel<K>, dim3(grid), dim3(block), lds, s, qrefs, meta, hp.nhubs, x.hub_items[K],
+                       hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax, hp.wmax, hp.rpc, H);
+    if (K >= 4) {
+        const uint32_t cgrid = std::min<uint32_t>((hp.nrows + 3) / 4, uint32_t(R.num_cus) * 8);
+        hipLaunchKernelGGL(hub_count_kernel<K>, dim3(cgrid), dim3(256), 0, s, meta, hp.nhubs, x.hub_rows[K], hp.nrows,
+                           cur + 1, x.d_result, x.slabs[K]);
     }
-    if (grid == 0) KZ_FAIL(KZ_ERR_STATE, "empty launch for bucket %d", gr.bucket);
-    if (global_rows) {
-        hipLaunchKernelGGL((clique_block_kernel<K, true>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
-                           cur, x.d_result, Dmax, W, H, hshift, slab);
-    } else {
-        hipLaunchKernelGGL((clique_block_kernel<K, false>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
-                           cur, x.d_result, Dmax, W, H, hshift, nullptr);
+    return KZ_OK;
+}
+
+// Hub plan for one k: per-hub metadata, item and row prefixes, HBM slab.
+static int prepare_hubs(CliqueJob& x, int k, const std::vector<uint32_t>& hv, const std::vector<uint32_t>& hq,
+                        const std::vector<uint32_t>& hd) {
+    CliqueJob::HubPlan hp;
+    hp.nhubs = uint32_t(hv.size());
+    hp.dmax = 0;
+    hp.wmax = 0;
+    for (uint32_t d : hd) {
+        hp.dmax = std::max(hp.dmax, d);
+        hp.wmax = std::max(hp.wmax, (d + 63) / 64);
     }
+    uint32_t H, hshift;
+    hash_size(hp.dmax, &H, &hshift);
+    const size_t lds_limit = rt().lds_per_block - kStaticLdsReserve;
+    hp.rpc = 64;
+    while (hp.rpc > 1 && hub_lds_bytes(k, hp.dmax, hp.wmax, hp.rpc, H) > lds_limit) hp.rpc >>= 1;
+    std::vector<HubMeta> meta(hp.nhubs);
+    std::vector<uint32_t> items(hp.nhubs), rows(hp.nhubs);
+    uint64_t slab = 0;
+    uint32_t ni = 0, nr = 0;
+    for (uint32_t h = 0; h < hp.nhubs; ++h) {
+        const uint32_t d = hd[h];
+        const uint32_t W = (d + 63) / 64;
+        meta[h] = HubMeta{hv[h], hq[h], d, W, slab};
+        slab += uint64_t(d) * W;
+        items[h] = ni;
+        rows[h] = nr;
+        ni += (d + hp.rpc - 1) / hp.rpc;
+        nr += d;
+    }
+    hp.nitems = ni;
+    hp.nrows = nr;
+    void* dm = nullptr;
+    uint32_t *di = nullptr, *dr = nullptr;
+    KZ_HIP(hipMalloc(&dm, sizeof(HubMeta) * std::max<uint32_t>(1, hp.nhubs)));
+    KZ_HIP(hipMalloc(&di, 4 * std::max<uint32_t>(1, hp.nhubs)));
+    KZ_HIP(hipMalloc(&dr, 4 * std::max<uint32_t>(1, hp.nhubs)));
+    x.hub_meta[k] = dm;
+    x.hub_items[k] = di;
+    x.hub_rows[k] = dr;
+    KZ_HIP(hipMemcpy(dm, meta.data(), sizeof(HubMeta) * meta.size(), hipMemcpyHostToDevice));
+    KZ_HIP(hipMemcpy(di, items.data(), 4 * items.size(), hipMemcpyHostToDevice));
+    KZ_HIP(hipMemcpy(dr, rows.data(), 4 * rows.size(), hipMemcpyHostToDevice));
+    if (k >= 4) {
+        unsigned long long* s = nullptr;
+        KZ_HIP(hipMalloc(&s, std::max<uint64_t>(1, slab) * 8));
+        x.slabs[k] = s;
+    }
+    x.hubplan[k] = hp;
     return KZ_OK;
 }
 
@@ -414,7 +640,6 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
     if (int e = set_lds_attrs()) return e;
     x.cfg = cfg;
     x.nqueries = uint32_t(queries.size());
-    // (k, bucket) -> list of (degree, vertex, query)
     struct T {
         uint32_t d, v, q;
     };
@@ -434,7 +659,6 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
     }
     std::vector<uint32_t> hv, hq;
     x.groups.clear();
-    x.ntasks = 0;
     int cursor = 0;
     for (auto& kv : lists) {
         auto& L = kv.second;
@@ -445,20 +669,22 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         gr.first = hv.size();
         gr.count = L.size();
         gr.dmax = gr.bucket == 0 ? 64 : (gr.bucket == kNumBuckets - 1 ? L.front().d : cfg.hi[gr.bucket]);
-        gr.cursor = cursor++;
+        gr.cursor = cursor;
+        cursor += (gr.bucket == kNumBuckets - 1) ? 2 : 1;  // hubs: rows + count cursors
         for (const T& t : L) {
             hv.push_back(t.v);
             hq.push_back(t.q);
         }
-        x.groups.push_back(gr);
-        if (gr.bucket == kNumBuckets - 1 && gr.k >= 4) {
-            const uint32_t W = (gr.dmax + 63) / 64;
-            const uint32_t blocks = uint32_t(std::min<uint64_t>(gr.count, uint64_t(rt().num_cus) * 2));
-            unsigned long long* slab = nullptr;
-            KZ_HIP(hipMalloc(&slab, size_t(blocks) * gr.dmax * W * 8));
-            x.slabs[gr.k] = slab;
-            x.slab_blocks[gr.k] = blocks;
+        if (gr.bucket == kNumBuckets - 1) {
+            std::vector<uint32_t> v, q, d;
+            for (const T& t : L) {
+                v.push_back(t.v);
+                q.push_back(t.q);
+                d.push_back(t.d);
+            }
+            if (int e = prepare_hubs(x, gr.k, v, q, d)) return e;
         }
+        x.groups.push_back(gr);
     }
     x.ntasks = hv.size();
     std::vector<QueryRef> qr(std::max<size_t>(1, queries.size()));
@@ -476,18 +702,19 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
     // capture
     Runtime& R = rt();
     hipStream_t main = R.stream;
-    const int nbr = std::max<int>(1, int(x.groups.size()));
-    std::vector<hipStream_t> br(static_cast<size_t>(nbr), nullptr);
-    std::vector<hipEvent_t> join(static_cast<size_t>(nbr), nullptr);
+    const size_t nbr = std::max<size_t>(1, x.groups.size());
+    std::vector<hipStream_t> br(nbr, nullptr);
+    std::vector<hipEvent_t> join(nbr, nullptr);
     hipEvent_t fork;
-    for (int i = 0; i < nbr; ++i) {
+    for (size_t i = 0; i < nbr; ++i) {
         KZ_HIP(hipStreamCreateWithFlags(&br[i], hipStreamNonBlocking));
         KZ_HIP(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
     }
     KZ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     KZ_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(reset_u32_kernel, dim3(1), dim3(256), 0, main, x.d_cursor, std::max(1, cursor));
-    hipLaunchKernelGGL(reset_u64_kernel, dim3(1), dim3(256), 0, main, x.d_result, int(std::max<size_t>(1, queries.size())));
+    hipLaunchKernelGGL(reset_u64_kernel, dim3(1), dim3(256), 0, main, x.d_result,
+                       int(std::max<size_t>(1, queries.size())));
     KZ_HIP(hipEventRecord(fork, main));
     int e = KZ_OK;
     for (size_t i = 0; i < x.groups.size() && e == KZ_OK; ++i) {
@@ -498,7 +725,6 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         else e = launch_group<5>(x, gr, br[i]);
     }
     for (size_t i = 0; i < x.groups.size(); ++i) {
-        if (i >= x.groups.size()) break;
         KZ_HIP(hipEventRecord(join[i], br[i]));
         KZ_HIP(hipStreamWaitEvent(main, join[i], 0));
     }
@@ -508,7 +734,7 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         e = KZ_ERR_DEVICE;
         kz::set_error("a bucket launch was rejected during capture");
     }
-    for (int i = 0; i < nbr; ++i) {
+    for (size_t i = 0; i < nbr; ++i) {
         hipStreamDestroy(br[i]);
         hipEventDestroy(join[i]);
     }

@@ -63,7 +63,7 @@ int ensure_forward(DeviceGraph& g) {
     std::vector<uint64_t> frp(size_t(g.n) + 1, 0);
     for (uint32_t v = 0; v < g.n; ++v) frp[v + 1] = frp[v] + g.h_fdeg[v];
     KZ_HIP(hipMalloc(&g.fwd_rowptr, sizeof(uint64_t) * frp.size()));
-    KZ_HIP(hipMalloc(&g.fwd_col, sizeof(uint32_t) * std::max<uint64_t>(1, frp[g.n])));
+    KZ_HIP(hipMalloc(&g.fwd_col, col_alloc_bytes(frp[g.n])));
     KZ_HIP(hipMemcpyAsync(g.fwd_rowptr, frp.data(), sizeof(uint64_t) * frp.size(), hipMemcpyHostToDevice, s));
     if (g.n)
         hipLaunchKernelGGL(fwd_copy_kernel, dim3((uint64_t(g.n) * 64 + 255) / 256), dim3(256), 0, s, g.rowptr, g.col,

@@ -32,6 +32,11 @@ struct Runtime {
 Runtime& rt();
 int ensure_ready();
 
+// Bytes allocated for a device column array of n entries: kernels read
+// neighbour lists as aligned 16-byte groups of 4 entries, so the last group
+// may extend up to 3 entries past the end (those entries are masked).
+inline size_t col_alloc_bytes(uint64_t n) { return sizeof(uint32_t) * (((n + 3) & ~uint64_t(3)) + 4); }
+
 // A CSR resident in HBM.  rowptr is u64 (graphs beyond 2^32 entries),
 // col and deg are u32.  `vbucket` holds vertex ids grouped by out-degree
 // bucket for the bitmap kernels (built lazily, see kz_clique.hip).
@@ -55,6 +60,9 @@ struct DeviceGraph {
     // prepared persistent clique executions, indexed by k (3..5), and the
     // degree-bucket configuration each uses (set or autotuned)
     std::shared_ptr<void> clique_exec[6];
+    // motif-census state: the graph relabelled by (degree, id) rank, its
+    // forward part, counter slabs and the 4-clique job (kz_census.hip)
+    std::shared_ptr<void> census;
     bool bucket_set[6] = {};
     uint32_t bucket_hi[6][5] = {};
     uint32_t bucket_block[6][5] = {};

@@ -45,16 +45,26 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t* keys, const uint16
     }
 }
 
+// Load through a pointer known to point at device (HBM) memory.  Graph
+// pointers read from query tables are generic to the compiler, which would
+// otherwise emit FLAT instructions; those wait on the LDS counter as well
+// and serialise latency hiding.  The cast lets it emit global loads.
+#define KZ_GLOBAL_AS __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p, uint64_t i) {
+    return ((const KZ_GLOBAL_AS T*)p)[i];
+}
+
 // Binary search for x in a sorted global list.
 __device__ __forceinline__ bool bsearch_global(const uint32_t* __restrict__ a, uint32_t n, uint32_t x) {
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t v = a[mid];
+        const uint32_t v = ldg(a, mid);
         if (v < x) lo = mid + 1;
         else hi = mid;
     }
-    return lo < n && a[lo] == x;
+    return lo < n && ldg(a, lo) == x;
 }
 
 // First index in a sorted list with a[i] >= x (lower bound).
@@ -62,7 +72,7 @@ __device__ __forceinline__ uint32_t lower_bound_global(const uint32_t* __restric
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < x) lo = mid + 1;
+        if (ldg(a, mid) < x) lo = mid + 1;
         else hi = mid;
     }
     return lo;
@@ -73,7 +83,7 @@ __device__ __forceinline__ uint32_t upper_bound_global(const uint32_t* __restric
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] <= x) lo = mid + 1;
+        if (ldg(a, mid) <= x) lo = mid + 1;
         else hi = mid;
     }
     return lo;
@@ -181,12 +191,29 @@ struct CliqueJob {
     uint32_t* d_task_q = nullptr;  // null for a single query
     uint32_t* d_cursor = nullptr;
     unsigned long long* d_result = nullptr;
+    // hub roots (last bucket), per k: split into row chunks over many
+    // workgroups; rows go to an HBM slab, then one wave counts each row
+    struct HubPlan {
+        uint32_t nhubs = 0, dmax = 0, wmax = 0, rpc = 64, nitems = 0, nrows = 0;
+    };
+    std::map<int, HubPlan> hubplan;
+    std::map<int, void*> hub_meta;         // HubMeta[nhubs]
+    std::map<int, uint32_t*> hub_items;    // exclusive prefix of row chunks per hub
+    std::map<int, uint32_t*> hub_rows;     // exclusive prefix of rows per hub
     std::map<int, unsigned long long*> slabs;  // per k, HBM bitmap rows
-    std::map<int, uint32_t> slab_blocks;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~CliqueJob();
 };
+
+// Forward part (entries > v) of sorted symmetric lists: per vertex the index
+// of the first entry > v and the forward degree; then a wave per vertex
+// copies the suffixes (kz_count.hip).
+__global__ void fwd_start_kernel(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col, uint32_t n,
+                                 uint32_t* __restrict__ skip, uint32_t* __restrict__ fdeg);
+__global__ void fwd_copy_kernel(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col, uint32_t n,
+                                const uint32_t* __restrict__ skip, const uint64_t* __restrict__ frowptr,
+                                uint32_t* __restrict__ fcol);
 
 int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, const BucketCfg& cfg);
 int clique_job_run(CliqueJob& x, uint64_t* counts, double* device_ms);

@@ -120,7 +120,7 @@ int kz_graph_upload(const kz_csr* h, kz_graph* out) {
     d.max_degree = h->max_degree;
     d.oriented = int(h->oriented);
     size_t rp_bytes = sizeof(uint64_t) * (size_t(d.n) + 1);
-    size_t col_bytes = sizeof(uint32_t) * (d.nnz ? d.nnz : 1);
+    size_t col_bytes = col_alloc_bytes(d.nnz);
     size_t deg_bytes = sizeof(uint32_t) * (d.n ? d.n : 1);
     auto fail = [&](const char* what) {
         hipFree(d.rowptr);

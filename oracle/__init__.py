@@ -69,6 +69,8 @@ def lib() -> C.CDLL:
         L.kzo_brute_motifs.restype = C.c_int
         L.kzo_clique_bytes.argtypes = [U64P, U32P, C.c_uint32, C.c_int, C.c_int, U32P, C.c_uint64, U64P]
         L.kzo_clique_bytes.restype = C.c_int
+        L.kzo_motif_census.argtypes = [U64P, U32P, C.c_uint32, C.c_int, U64P, U64P]
+        L.kzo_motif_census.restype = C.c_int
         _lib = L
     return _lib
 
@@ -164,6 +166,19 @@ def brute_motifs(rowptr, col) -> list[int]:
     if lib().kzo_brute_motifs(_p64(rp), _p32(cl), len(rp) - 1, _p64(out)) < 0:
         raise ValueError("graph too large for brute force")
     return [int(x) for x in out]
+
+
+def motif_census(rowptr, col, threads: int = 0, non_induced: bool = False):
+    """Induced census [wedge, triangle, 3-path, 3-star, 4-cycle, tailed-triangle,
+    diamond, 4-clique] of a symmetric graph (combinatorial, any size); with
+    non_induced=True also returns the non-induced counts in the same order."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.uint64)
+    cl = _u32(col) if len(col) else np.zeros(1, np.uint32)
+    out = np.zeros(8, np.uint64)
+    ni = np.zeros(8, np.uint64)
+    lib().kzo_motif_census(_p64(rp), _p32(cl), len(rp) - 1, threads, _p64(out), _p64(ni))
+    res = [int(x) for x in out]
+    return (res, [int(x) for x in ni]) if non_induced else res
 
 
 def clique_bytes(rowptr, col, kmax: int = 4, threads: int = 0, roots=None) -> dict:

@@ -692,4 +692,127 @@ int kzo_clique_bytes(const uint64_t* rowptr, const uint32_t* col, uint32_t n, in
     return 0;
 }
 
+// ------------------------------------------------------------ motif census --
+// Induced connected 3- and 4-vertex census of a symmetric graph (sorted
+// adjacency, no loops or duplicates), in the order of kzo_brute_motifs:
+// {wedge, triangle, 3-path, 3-star, 4-cycle, tailed triangle, diamond,
+// 4-clique}.  The reference has no 4-vertex census app; this follows the
+// standard combinatorial route.  Non-induced counts come from degrees d,
+// per-edge triangle counts t(e) = |N(u) & N(v)|, per-vertex triangle counts
+// t(v), 4-cycles and 4-cliques:
+//   wedges = sum C(d,2)          3-stars = sum C(d,3)
+//   3-paths = sum_e (d_u-1)(d_v-1) - 3T      tailed = sum_v t(v) (d_v - 2)
+//   diamonds = sum_e C(t(e),2)   4-cycles: each counted once from its
+//   highest-ranked vertex u (rank = (degree, id)) as C(c(w), 2) over the
+//   vertices w reached by u -> v -> w with v, w ranked below u.
+// Induced counts follow by back-substitution through the containment
+// matrix of the six connected 4-vertex graphs (each row: copies of that
+// graph inside 3-path, 3-star, 4-cycle, tailed, diamond, 4-clique):
+//   3-path  1 0 4 2 6 12 | 3-star 0 1 0 1 2 4 | 4-cycle 0 0 1 0 1 3
+//   tailed  0 0 0 1 4 12 | diamond 0 0 0 0 1 6 | 4-clique 0 0 0 0 0 1
+// nonind (may be NULL) receives the non-induced counts in the same order.
+// All sums wrap mod 2^64, so the differences are exact.
+int kzo_motif_census(const uint64_t* rowptr, const uint32_t* col, uint32_t n, int threads, uint64_t* out,
+                     uint64_t* nonind) {
+    if (threads > 0) omp_set_num_threads(threads);
+    auto deg = [&](uint32_t v) { return uint64_t(rowptr[v + 1] - rowptr[v]); };
+    std::vector<uint32_t> order(n), rank(n);
+    for (uint32_t v = 0; v < n; ++v) order[v] = v;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return deg(a) != deg(b) ? deg(a) < deg(b) : a < b;
+    });
+    for (uint32_t i = 0; i < n; ++i) rank[order[i]] = i;
+    // rank-oriented out-lists (kept in id order) for the 4-cliques
+    std::vector<uint64_t> orp(size_t(n) + 1, 0);
+    for (uint32_t v = 0; v < n; ++v) {
+        uint64_t c = 0;
+        for (uint64_t e = rowptr[v]; e < rowptr[v + 1]; ++e) c += rank[col[e]] > rank[v];
+        orp[v + 1] = orp[v] + c;
+    }
+    std::vector<uint32_t> ocol(orp[n] ? orp[n] : 1);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t vi = 0; vi < int64_t(n); ++vi) {
+        const uint32_t v = uint32_t(vi);
+        uint64_t o = orp[v];
+        for (uint64_t e = rowptr[v]; e < rowptr[v + 1]; ++e)
+            if (rank[col[e]] > rank[v]) ocol[o++] = col[e];
+    }
+    std::vector<uint64_t> tv(n, 0);  // sum of t(e) over the edges at v (= 2 t(v))
+    uint64_t S2 = 0, S3 = 0, P3 = 0, TE = 0, DIA = 0, C4 = 0, K4 = 0;
+#pragma omp parallel reduction(+ : S2, S3, P3, TE, DIA, C4, K4)
+    {
+        std::vector<uint32_t> cnt(n, 0), touched, p1;
+        touched.reserve(1024);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t ui = 0; ui < int64_t(n); ++ui) {
+            const uint32_t u = uint32_t(ui);
+            const uint64_t du = deg(u);
+            S2 += du * (du - 1) / 2;
+            S3 += du * (du - 1) * (du - 2) / 6;
+            const uint32_t* Nu = col + rowptr[u];
+            for (uint64_t e = rowptr[u]; e < rowptr[u + 1]; ++e) {
+                const uint32_t v = col[e];
+                if (v <= u) continue;
+                const uint64_t dv = deg(v);
+                const uint64_t t = kzo_intersect_count(Nu, uint32_t(du), col + rowptr[v], uint32_t(dv));
+                TE += t;
+                DIA += t * (t - 1) / 2;
+                P3 += (du - 1) * (dv - 1);
+#pragma omp atomic
+                tv[u] += t;
+#pragma omp atomic
+                tv[v] += t;
+            }
+            // 4-cycles with u ranked highest
+            for (uint64_t e = rowptr[u]; e < rowptr[u + 1]; ++e) {
+                const uint32_t v = col[e];
+                if (rank[v] > rank[u]) continue;
+                for (uint64_t f = rowptr[v]; f < rowptr[v + 1]; ++f) {
+                    const uint32_t w = col[f];
+                    if (rank[w] >= rank[u]) continue;
+                    if (cnt[w]++ == 0) touched.push_back(w);
+                }
+            }
+            for (uint32_t w : touched) {
+                C4 += uint64_t(cnt[w]) * (cnt[w] - 1) / 2;
+                cnt[w] = 0;
+            }
+            touched.clear();
+            // 4-cliques u < v < w < x in rank order
+            const uint32_t* Ou = ocol.data() + orp[u];
+            const uint32_t ou = uint32_t(orp[u + 1] - orp[u]);
+            p1.resize(ou);
+            for (uint32_t i = 0; i < ou; ++i) {
+                const uint32_t v = Ou[i];
+                const uint32_t c = kzo_intersect(Ou, ou, ocol.data() + orp[v], uint32_t(orp[v + 1] - orp[v]), p1.data());
+                for (uint32_t j = 0; j < c; ++j) {
+                    const uint32_t w = p1[j];
+                    K4 += kzo_intersect_count(p1.data(), c, ocol.data() + orp[w], uint32_t(orp[w + 1] - orp[w]));
+                }
+            }
+        }
+    }
+    uint64_t PAW = 0;
+    for (uint32_t v = 0; v < n; ++v) PAW += (tv[v] / 2) * (deg(v) >= 2 ? deg(v) - 2 : 0);
+    const uint64_t T = TE / 3;
+    const uint64_t N[8] = {S2, T, P3 - 3 * T, S3, C4, PAW, DIA, K4};
+    if (nonind)
+        for (int i = 0; i < 8; ++i) nonind[i] = N[i];
+    const uint64_t iK4 = N[7];
+    const uint64_t iDia = N[6] - 6 * iK4;
+    const uint64_t iPaw = N[5] - 4 * iDia - 12 * iK4;
+    const uint64_t iC4 = N[4] - iDia - 3 * iK4;
+    const uint64_t iStar = N[3] - iPaw - 2 * iDia - 4 * iK4;
+    const uint64_t iP3 = N[2] - 4 * iC4 - 2 * iPaw - 6 * iDia - 12 * iK4;
+    out[0] = S2 - 3 * T;
+    out[1] = T;
+    out[2] = iP3;
+    out[3] = iStar;
+    out[4] = iC4;
+    out[5] = iPaw;
+    out[6] = iDia;
+    out[7] = iK4;
+    return 8;
+}
+
 }  // extern "C"

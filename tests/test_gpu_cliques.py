@@ -104,3 +104,29 @@ def test_autotuner_keeps_counts_exact(kzdev):
     with pytest.raises(kz.KzError):
         dg.set_buckets(4, [128, 256, 512, 1024], [256, 256, 256, 512, 512])
     dg.release()
+
+
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_hub_and_bucket_paths_exact(kzdev, k):
+    """Every bucket path (wave, LDS rows, hub rows in HBM + per-row count) on
+    the same graphs: default bounds, bounds that push most roots into the hub
+    kernels, and a dense graph whose forward degrees exceed 1024."""
+    kz = kzdev
+    cases = [("rmat12", kz.gen_rmat(12, 16, seed=21).oriented()),
+             ("er_dense", kz.gen_erdos_renyi(600, 40000, 22).oriented())]
+    # two planted hubs with forward degree > 1024 on a sparse background
+    import numpy as np
+    rng = np.random.default_rng(23)
+    src = np.concatenate([rng.integers(0, 3000, 60000), np.zeros(2000, np.int64), np.ones(1500, np.int64)])
+    dst = np.concatenate([rng.integers(0, 3000, 60000), np.arange(1, 2001), np.arange(2, 1502)])
+    cases.append(("planted_hubs", kz.from_edges(src, dst, 3000)))
+    for name, g in cases:
+        print(f"[case] graph={name} k={k} max_degree={g.max_degree}", flush=True)
+        want = oracle.run_app(APP[k], g.rowptr, g.col)[1][0]
+        dg = g.to_device()
+        assert dg.count_cliques(k, "persistent") == want, (name, "default")
+        dg.set_buckets(k, [16, 32, 48, 64], [256, 256, 256, 512, 512])
+        assert dg.count_cliques(k, "persistent") == want, (name, "hub-heavy")
+        dg.set_buckets(k, [8, 24, 40, 56], [256, 512, 512, 512, 256])
+        assert dg.count_cliques(k, "persistent") == want, (name, "hub-heavy-256")
+        dg.release()

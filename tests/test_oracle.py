@@ -148,3 +148,71 @@ def test_motif_brute_force_identities():
     assert k4 == oracle.brute_cliques(g.rowptr, g.col, 4)
     assert t == oracle.brute_cliques(g.rowptr, g.col, 3)
     assert w + 3 * t == int((d * (d - 1) // 2).sum())
+
+
+def _sym(n, edges):
+    src = [a for a, b in edges] + [b for a, b in edges]
+    dst = [b for a, b in edges] + [a for a, b in edges]
+    return kz.from_edges(np.array(src, np.uint32), np.array(dst, np.uint32), n)
+
+
+def test_census_against_brute_force():
+    """The combinatorial census (any size) against exhaustive enumeration of
+    every 3- and 4-subset on random graphs from empty to complete."""
+    rng = np.random.default_rng(3)
+    for _ in range(120):
+        n = int(rng.integers(4, 48))
+        m = int(rng.integers(1, n * (n - 1) // 2 + 1))
+        s, t = rng.integers(0, n, m), rng.integers(0, n, m)
+        g = kz.from_edges(np.concatenate([s, t]), np.concatenate([t, s]), n)
+        if g.num_vertices == 0:
+            continue
+        assert oracle.motif_census(g.rowptr, g.col) == oracle.brute_motifs(g.rowptr, g.col)
+
+
+def test_census_closed_forms():
+    """Each pattern alone, K_n, stars and cycles: induced counts by hand."""
+    order = ["wedge", "triangle", "3-path", "3-star", "4-cycle", "tailed", "diamond", "4-clique"]
+
+    def census(n, edges):
+        g = _sym(n, edges)
+        return dict(zip(order, oracle.motif_census(g.rowptr, g.col)))
+
+    only = {
+        "3-path": (4, [(0, 1), (1, 2), (2, 3)]),
+        "3-star": (4, [(0, 1), (0, 2), (0, 3)]),
+        "4-cycle": (4, [(0, 1), (1, 2), (2, 3), (3, 0)]),
+        "tailed": (4, [(0, 1), (1, 2), (2, 0), (2, 3)]),
+        "diamond": (4, [(0, 1), (1, 2), (2, 0), (1, 3), (2, 3)]),
+        "4-clique": (4, [(a, b) for a in range(4) for b in range(a + 1, 4)]),
+    }
+    for name, (n, edges) in only.items():
+        c = census(n, edges)
+        assert all(c[k] == (1 if k == name else 0) for k in order[2:]), (name, c)
+    for n in (5, 9, 17):
+        c = census(n, [(a, b) for a in range(n) for b in range(a + 1, n)])
+        assert c["4-clique"] == comb(n, 4) and c["triangle"] == comb(n, 3)
+        assert sum(c[k] for k in order) == comb(n, 4) + comb(n, 3)
+        c = census(n, [(0, b) for b in range(1, n)])
+        assert c["3-star"] == comb(n - 1, 3) and c["wedge"] == comb(n - 1, 2)
+        c = census(n, [(a, (a + 1) % n) for a in range(n)])
+        assert c["3-path"] == n and c["wedge"] == n and c["4-cycle"] == 0
+
+
+def test_census_non_induced_identities():
+    """Non-induced counts against independent formulas on a Chung-Lu graph:
+    stars from degrees, 4-cliques from the clique oracle, and the induced
+    counts re-expanded through the containment matrix."""
+    g = kz.gen_chung_lu(3000, 15000, 2.3, 200, 5)
+    ind, non = oracle.motif_census(g.rowptr, g.col, non_induced=True)
+    d = np.diff(g.rowptr).astype(np.int64)
+    o = g.oriented()
+    assert non[3] == int((d * (d - 1) * (d - 2) // 6).sum())
+    assert non[0] == int((d * (d - 1) // 2).sum())
+    assert non[7] == oracle.run_app("kautomine_four_clique", o.rowptr, o.col)[1][0]
+    assert non[1] == oracle.run_app("kautomine_triangle", o.rowptr, o.col)[1][0]
+    w, t, p3, st, c4, paw, dia, k4 = ind
+    A = [[1, 0, 4, 2, 6, 12], [0, 1, 0, 1, 2, 4], [0, 0, 1, 0, 1, 3], [0, 0, 0, 1, 4, 12], [0, 0, 0, 0, 1, 6],
+         [0, 0, 0, 0, 0, 1]]
+    x = [p3, st, c4, paw, dia, k4]
+    assert [sum(A[i][j] * x[j] for j in range(6)) for i in range(6)] == non[2:]
