@@ -172,12 +172,15 @@ int kz_count_cliques(kz_graph g, int k, int method, uint64_t* count, kz_run_stat
 /* Degree buckets of the persistent form: root tasks with |N+(u)| <= bound[0]
  * run one per wave (bound[0] <= 64); up to bound[1..3] one per workgroup
  * with bitmap rows in LDS (bound[3] <= 1024); larger ones keep rows in HBM.
- * block[b] = threads per workgroup of bucket b (256 or 512; bucket 0: 256).
+ * block[b] = threads per workgroup of bucket b (256, 512 or 1024; bucket 0:
+ * 256).  serial != 0 runs the bucket kernels one after another, each with
+ * the whole GPU, instead of as concurrent branches of the captured graph.
  * No reference counterpart: the reference balances roots with 64-embedding
  * chunks per thread (src/engine.cc:985-989). */
 typedef struct kz_bucket_cfg {
     uint32_t bound[4];
     uint32_t block[5];
+    uint32_t serial;
 } kz_bucket_cfg;
 int kz_clique_get_buckets(kz_graph g, int k, kz_bucket_cfg* out);
 int kz_clique_set_buckets(kz_graph g, int k, const kz_bucket_cfg* cfg);

@@ -264,12 +264,13 @@ class DeviceGraph:
         check(lib().kz_clique_get_buckets(self._h, k, C.byref(cfg)))
         return cfg.as_dict()
 
-    def set_buckets(self, k: int, bound, block) -> None:
+    def set_buckets(self, k: int, bound, block, serial: bool = False) -> None:
         cfg = _ffi.kz_bucket_cfg()
         for i, x in enumerate(bound):
             cfg.bound[i] = x
         for i, x in enumerate(block):
             cfg.block[i] = x
+        cfg.serial = 1 if serial else 0
         check(lib().kz_clique_set_buckets(self._h, k, C.byref(cfg)))
 
     def clique_workload(self) -> dict:

@@ -103,10 +103,10 @@ class kz_run_stats(C.Structure):
 
 
 class kz_bucket_cfg(C.Structure):
-    _fields_ = [("bound", C.c_uint32 * 4), ("block", C.c_uint32 * 5)]
+    _fields_ = [("bound", C.c_uint32 * 4), ("block", C.c_uint32 * 5), ("serial", C.c_uint32)]
 
     def as_dict(self) -> dict:
-        return {"bound": list(self.bound), "block": list(self.block)}
+        return {"bound": list(self.bound), "block": list(self.block), "serial": int(self.serial)}
 
 
 class kz_set_ref(C.Structure):

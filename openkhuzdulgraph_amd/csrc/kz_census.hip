@@ -131,14 +131,14 @@ __global__ void fill_small_kernel(uint32_t* __restrict__ a, int na, unsigned lon
 // Pass 1: triangles per forward edge.  One root per workgroup of T threads
 // (T = 64: roots with d+ <= 64, one wave).  Dynamic LDS: [row descriptors
 // Dmax u64][L Dmax u32][edge counters Dmax u32][group prefix Dmax+1 u32]
-// [hash keys H u32][hash vals H u16].
+// [filter FW u32][hash keys H u32][hash vals H u16].
 template <int T>
 __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restrict__ frowptr,
                                                        const uint32_t* __restrict__ fcol,
                                                        const uint32_t* __restrict__ tasks, uint32_t ntasks,
                                                        uint32_t* __restrict__ cursor, uint32_t* __restrict__ troot,
                                                        uint32_t* __restrict__ tmid, uint32_t Dmax, uint32_t H,
-                                                       uint32_t hshift) {
+                                                       uint32_t hshift, uint32_t FW, uint32_t fshift) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char* p = smem;
     unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
@@ -149,13 +149,15 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
     p += size_t(Dmax) * 4;
     uint32_t* rp = reinterpret_cast<uint32_t*>(p);
     p += size_t(Dmax + 1) * 4;
+    uint32_t* fb = reinterpret_cast<uint32_t*>(p);
+    p += size_t(FW) * 4;
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);
     p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
     __shared__ uint32_t sTask, sTmp[T / 64];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t hmask = H - 1;
+    const RootSet rs{hk, hv, H - 1, hshift, fb, fshift};
     while (true) {
         if (threadIdx.x == 0) sTask = atomicAdd(cursor, 1u);
         __syncthreads();
@@ -168,9 +170,9 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
             L[i] = ldg(fcol, ub + i);
             cnt[i] = 0;
         }
-        for (uint32_t i = threadIdx.x; i < H; i += T) hk[i] = kEmpty;
+        root_clear(rs, H, FW, threadIdx.x, T);
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < d; i += T) hash_insert(hk, hv, hmask, hshift, L[i], uint16_t(i));
+        for (uint32_t i = threadIdx.x; i < d; i += T) root_insert(rs, L[i], uint16_t(i));
         const uint32_t nr = d - 1;
         for (uint32_t r = threadIdx.x; r <= nr; r += T) {
             uint32_t groups = 0;
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
             atomicAdd(&cnt[j], 1u);
             atomicAdd(&tmid[at], 1u);
         };
-        stream_groups(fcol, rb, rp, nr, G, threadIdx.x, T, hk, hv, hmask, hshift, L[d - 1], hit);
+        stream_groups(fcol, rb, rp, nr, G, threadIdx.x, T, rs, L[d - 1], hit);
         // skewed rows: one wave per row binary-searches the candidates
         for (uint32_t r = wave; r < nr; r += T / 64) {
             const uint64_t ds = rb[r];
@@ -295,7 +297,9 @@ static void hash_geometry(uint32_t Dmax, uint32_t* H, uint32_t* hshift) {
 }
 
 static size_t tri_lds_bytes(uint32_t Dmax, uint32_t H) {
-    return size_t(Dmax) * 8 + size_t(Dmax) * 8 + size_t(Dmax + 1) * 4 + size_t(H) * 6 + 16;
+    uint32_t fw, fs;
+    filter_geometry(Dmax, &fw, &fs);
+    return size_t(Dmax) * 8 + size_t(Dmax) * 8 + size_t(Dmax + 1) * 4 + size_t(fw) * 4 + size_t(H) * 6 + 16;
 }
 
 static int sort_u64(uint64_t* in, uint64_t* out, uint64_t m, int end_bit, Scratch& tmp, hipStream_t s) {
@@ -467,9 +471,11 @@ static int run_census(Ranked& R, uint64_t* sums, double* device_ms) {
     if (R.nsmall) {
         uint32_t H, hs;
         hash_geometry(kSmallRoot, &H, &hs);
+        uint32_t fw, fs;
+        filter_geometry(kSmallRoot, &fw, &fs);
         const uint32_t grid = std::min<uint32_t>(R.nsmall, uint32_t(rt_.num_cus) * 32);
         hipLaunchKernelGGL(census_tri_kernel<64>, dim3(grid), dim3(64), tri_lds_bytes(kSmallRoot, H), s, R.frowptr,
-                           R.fcol, R.tasks, R.nsmall, R.cursor, R.troot, R.tmid, kSmallRoot, H, hs);
+                           R.fcol, R.tasks, R.nsmall, R.cursor, R.troot, R.tmid, kSmallRoot, H, hs, fw, fs);
     }
     if (R.nlarge) {
         uint32_t H, hs;
@@ -483,8 +489,10 @@ static int run_census(Ranked& R, uint64_t* sums, double* device_ms) {
         uint32_t per_cu = std::max<uint32_t>(1, uint32_t(rt_.lds_per_block / (lds + kStaticLdsReserve)));
         per_cu = std::min<uint32_t>(per_cu, 8);
         const uint32_t grid = std::min<uint32_t>(R.nlarge, uint32_t(rt_.num_cus) * per_cu);
+        uint32_t fw, fs;
+        filter_geometry(R.large_dmax, &fw, &fs);
         hipLaunchKernelGGL(census_tri_kernel<256>, dim3(grid), dim3(256), lds, s, R.frowptr, R.fcol,
-                           R.tasks + R.nsmall, R.nlarge, R.cursor + 1, R.troot, R.tmid, R.large_dmax, H, hs);
+                           R.tasks + R.nsmall, R.nlarge, R.cursor + 1, R.troot, R.tmid, R.large_dmax, H, hs, fw, fs);
     }
     hipLaunchKernelGGL(census_sums_kernel, dim3(std::max(1, rt_.num_cus * 4)), dim3(256), 0, s, R.frowptr, R.fcol,
                        R.deg, R.n, R.troot, R.tmid, R.sums);

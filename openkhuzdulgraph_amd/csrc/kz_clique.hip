@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
     __shared__ uint32_t sL[WPB][64];
     __shared__ uint32_t sKey[WPB][H];
     __shared__ uint16_t sVal[WPB][H];
+    __shared__ uint32_t sFb[WPB][32];  // 1024-bit filter: 16 bits per id
     __shared__ unsigned long long sRow[WPB][64];
     __shared__ unsigned long long sRb[WPB][64];
     __shared__ uint32_t sRp[WPB][64];
@@ -83,6 +84,7 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
     uint32_t* L = sL[w];
     uint32_t* hk = sKey[w];
     uint16_t* hv = sVal[w];
+    const RootSet rs{hk, hv, H - 1, 25, sFb[w], 22};
     unsigned long long* rows = sRow[w];
     unsigned long long* rb = sRb[w];
     uint32_t* rp = sRp[w];
@@ -109,6 +111,7 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
         L[lane] = x;
         hk[lane] = kEmpty;
         hk[lane + 64] = kEmpty;
+        if (lane < 32) rs.fb[lane] = 0;
         rows[lane] = 0;
         // row descriptors: lane r owns row r
         // (rows >= d - 1 are empty, so rp[d - 1] is the group total)
@@ -122,10 +125,10 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
         rp[lane] = excl;
         const uint64_t skmask = __ballot(skewed);
         wave_lds_sync();
-        if (lane < d) hash_insert(hk, hv, H - 1, 25, x, uint16_t(lane));
+        if (lane < d) root_insert(rs, x, uint16_t(lane));
         wave_lds_sync();
         const uint32_t lmax = L[d - 1];
-        stream_groups(col, rb, rp, d - 1, G, lane, 64, hk, hv, H - 1, 25, lmax,
+        stream_groups(col, rb, rp, d - 1, G, lane, 64, rs, lmax,
                       [&](uint32_t r, uint32_t j, uint64_t) {
                           if (K == 3) acc += 1;
                           else atomicOr(&rows[r], 1ull << j);
@@ -212,16 +215,14 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
     }
 }
 
-// Builds rows [r0, r1) of the DAG of root with list L (d ids, hash hk/hv)
-// into rows_out (word kk of row r at rows_out[kk * ws + r - r0], zeroed by
-// the caller),
-// streaming the neighbour lists of all rows as 16-byte groups.  rb holds
+// Builds rows [r0, r1) of the DAG of root with list L (d ids, set rs) into
+// rows_out (word kk of row r at rows_out[kk * ws + r - r0], zeroed by the
+// caller), streaming the neighbour lists of all rows as 16-byte groups.  rb holds
 // (r1 - r0) row descriptors and rp (r1 - r0 + 1) group prefixes, both LDS
 // scratch; tmp holds one u32 per wave.  Block-wide.
 template <int K>
 __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
-                                                 const uint32_t* L, uint32_t d, const uint32_t* hk, const uint16_t* hv,
-                                                 uint32_t hmask, uint32_t hshift, uint32_t r0, uint32_t r1,
+                                                 const uint32_t* L, uint32_t d, const RootSet& rs, uint32_t r0, uint32_t r1,
                                                  unsigned long long* rows_out, uint32_t ws, unsigned long long* rb,
                                                  uint32_t* rp, uint32_t* tmp, unsigned long long& acc) {
     const uint32_t lane = threadIdx.x & 63;
@@ -238,7 +239,7 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
     }
     __syncthreads();
     const uint32_t G = block_exclusive_scan(rp, nr + 1, tmp);
-    stream_groups(col, rb, rp, nr, G, threadIdx.x, blockDim.x, hk, hv, hmask, hshift, L[d - 1],
+    stream_groups(col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
                   [&](uint32_t r, uint32_t j, uint64_t) {
                       if (K == 3) acc += 1;
                       else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
@@ -263,14 +264,15 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
 // ----------------------------------------------------------- block tasks --
 // One task per workgroup, rows in LDS.  Dynamic LDS: [rows Dmax*W u64 (k>=4)]
 // [row descriptors Dmax u64][L Dmax u32][group prefix Dmax+1 u32]
-// [hash keys H u32][hash vals H u16].
+// [filter FW u32][hash keys H u32][hash vals H u16].
 template <int K>
-__global__ __launch_bounds__(512) void clique_block_kernel(const QueryRef* __restrict__ queries,
+__global__ __launch_bounds__(1024) void clique_block_kernel(const QueryRef* __restrict__ queries,
                                                            const uint32_t* __restrict__ task_v,
                                                            const uint32_t* __restrict__ task_q, uint32_t ntasks,
                                                            uint32_t* __restrict__ cursor,
                                                            unsigned long long* __restrict__ results, uint32_t Dmax,
-                                                           uint32_t W, uint32_t H, uint32_t hshift) {
+                                                           uint32_t W, uint32_t H, uint32_t hshift, uint32_t FW,
+                                                           uint32_t fshift) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char* p = smem;
     unsigned long long* rows = nullptr;
@@ -284,13 +286,15 @@ __global__ __launch_bounds__(512) void clique_block_kernel(const QueryRef* __res
     p += size_t(Dmax) * 4;
     uint32_t* rp = reinterpret_cast<uint32_t*>(p);
     p += size_t(Dmax + 1) * 4;
+    uint32_t* fb = reinterpret_cast<uint32_t*>(p);
+    p += size_t(FW) * 4;
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);
     p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
-    __shared__ uint32_t sTask, sTmp[8];
+    __shared__ uint32_t sTask, sTmp[16];
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
-    const uint32_t hmask = H - 1;
+    const RootSet rs{hk, hv, H - 1, hshift, fb, fshift};
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
     const uint64_t* rowptr = nullptr;
@@ -311,13 +315,13 @@ __global__ __launch_bounds__(512) void clique_block_kernel(const QueryRef* __res
         const uint64_t ub = ldg(rowptr, u);
         const uint32_t d = uint32_t(ldg(rowptr, u + 1) - ub);
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) L[i] = ldg(col, ub + i);
-        for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) hk[i] = kEmpty;
+        root_clear(rs, H, FW, threadIdx.x, blockDim.x);
         if (K >= 4)
             for (uint32_t i = threadIdx.x; i < d * W; i += blockDim.x) rows[i] = 0;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) hash_insert(hk, hv, hmask, hshift, L[i], uint16_t(i));
+        for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
         __syncthreads();
-        build_rows_block<K>(rowptr, col, L, d, hk, hv, hmask, hshift, 0, d, rows, d, rb, rp, sTmp, acc);
+        build_rows_block<K>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, acc);
         if (K >= 4) count_rows<K>(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
         __syncthreads();
     }
@@ -331,13 +335,13 @@ __global__ __launch_bounds__(512) void clique_block_kernel(const QueryRef* __res
 // [row descriptors RPC u64][L Dmax u32][group prefix RPC+1 u32]
 // [hash keys H u32][hash vals H u16].
 template <int K>
-__global__ __launch_bounds__(512) void hub_rows_kernel(const QueryRef* __restrict__ queries,
+__global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restrict__ queries,
                                                        const HubMeta* __restrict__ hubs, uint32_t nhubs,
                                                        const uint32_t* __restrict__ item_prefix, uint32_t nitems,
                                                        uint32_t* __restrict__ cursor,
                                                        unsigned long long* __restrict__ results,
                                                        unsigned long long* __restrict__ slab, uint32_t Dmax,
-                                                       uint32_t Wmax, uint32_t RPC, uint32_t H) {
+                                                       uint32_t Wmax, uint32_t RPC, uint32_t H, uint32_t FW) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char* p = smem;
     unsigned long long* rbuf = nullptr;
@@ -351,16 +355,18 @@ __global__ __launch_bounds__(512) void hub_rows_kernel(const QueryRef* __restric
     p += size_t(Dmax) * 4;
     uint32_t* rp = reinterpret_cast<uint32_t*>(p);
     p += size_t(RPC + 1) * 4;
+    uint32_t* fb = reinterpret_cast<uint32_t*>(p);
+    p += size_t(FW) * 4;
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);
     p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
-    __shared__ uint32_t sItem, sTmp[8];
+    __shared__ uint32_t sItem, sTmp[16];
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
     uint32_t hcur = 0xFFFFFFFFu;
     const uint64_t* rowptr = nullptr;
     const uint32_t* col = nullptr;
-    uint32_t hmask = 0, hshift = 0;
+    RootSet rs{hk, hv, 0, 0, fb, 0};
     while (true) {
         if (threadIdx.x == 0) sItem = atomicAdd(cursor, 1u);
         __syncthreads();
@@ -380,13 +386,15 @@ __global__ __launch_bounds__(512) void hub_rows_kernel(const QueryRef* __restric
             while (Hh < 2 * d) Hh <<= 1;
             uint32_t bits = 0;
             while ((1u << bits) < Hh) ++bits;
-            hmask = Hh - 1;
-            hshift = 32 - bits;
+            uint32_t fw = 0;
+            rs.hmask = Hh - 1;
+            rs.hshift = 32 - bits;
+            filter_geometry(d, &fw, &rs.fshift);
             const uint64_t ub = ldg(rowptr, hm.u);
             for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) L[i] = ldg(col, ub + i);
-            for (uint32_t i = threadIdx.x; i < Hh; i += blockDim.x) hk[i] = kEmpty;
+            root_clear(rs, Hh, fw, threadIdx.x, blockDim.x);
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) hash_insert(hk, hv, hmask, hshift, L[i], uint16_t(i));
+            for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
             hcur = h;
         }
         const uint32_t r0 = (it - item_prefix[h]) * RPC;
@@ -394,7 +402,7 @@ __global__ __launch_bounds__(512) void hub_rows_kernel(const QueryRef* __restric
         if (K >= 4)
             for (uint32_t i = threadIdx.x; i < (r1 - r0) * hm.W; i += blockDim.x) rbuf[i] = 0;
         __syncthreads();
-        build_rows_block<K>(rowptr, col, L, d, hk, hv, hmask, hshift, r0, r1, rbuf, r1 - r0, rb, rp, sTmp, acc);
+        build_rows_block<K>(rowptr, col, L, d, rs, r0, r1, rbuf, r1 - r0, rb, rp, sTmp, acc);
         if (K >= 4) {
             // the slab is word-major over all d rows of the hub
             const uint32_t nr = r1 - r0;
@@ -447,12 +455,15 @@ __global__ void reset_u64_kernel(unsigned long long* p, int n) {
 
 BucketCfg default_buckets() {
     BucketCfg c;
-    const uint32_t hi[kNumBuckets] = {64, 256, 512, 1024, 0xFFFFFFFFu};
-    const uint32_t blk[kNumBuckets] = {256, 256, 256, 512, 512};
+    // measured best on power-law graphs (RMAT scale 22): the LDS-heavy
+    // buckets in 1024-thread workgroups, bucket kernels one after another
+    const uint32_t hi[kNumBuckets] = {64, 128, 512, 1024, 0xFFFFFFFFu};
+    const uint32_t blk[kNumBuckets] = {256, 256, 512, 1024, 1024};
     for (int b = 0; b < kNumBuckets; ++b) {
         c.hi[b] = hi[b];
         c.block[b] = blk[b];
     }
+    c.serial = 1;
     return c;
 }
 
@@ -462,7 +473,8 @@ int check_buckets(const BucketCfg& c) {
         if (c.hi[b] <= c.hi[b - 1]) KZ_FAIL(KZ_ERR_INVALID, "bucket bounds must increase");
     if (c.hi[kNumBuckets - 2] > 1024) KZ_FAIL(KZ_ERR_INVALID, "LDS buckets end at 1024 or below");
     for (int b = 0; b < kNumBuckets; ++b)
-        if (c.block[b] != 256 && c.block[b] != 512) KZ_FAIL(KZ_ERR_INVALID, "block sizes must be 256 or 512");
+        if (c.block[b] != 256 && c.block[b] != 512 && c.block[b] != 1024)
+            KZ_FAIL(KZ_ERR_INVALID, "block sizes must be 256, 512 or 1024");
     if (c.block[0] != 256) KZ_FAIL(KZ_ERR_INVALID, "the wave bucket runs 256-thread blocks");
     return KZ_OK;
 }
@@ -488,14 +500,20 @@ CliqueJob::~CliqueJob() {
 }
 
 // Must match the carving at the top of clique_block_kernel / hub_rows_kernel.
+static uint32_t filter_words(uint32_t Dmax) {
+    uint32_t w, sh;
+    filter_geometry(Dmax, &w, &sh);
+    return w;
+}
+
 static size_t block_lds_bytes(int K, uint32_t Dmax, uint32_t W, uint32_t H) {
     return (K >= 4 ? size_t(Dmax) * W * 8 : 0) + size_t(Dmax) * 8 + size_t(Dmax) * 4 + size_t(Dmax + 1) * 4 +
-           size_t(H) * 6 + 16;
+           size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
 }
 
 static size_t hub_lds_bytes(int K, uint32_t Dmax, uint32_t Wmax, uint32_t RPC, uint32_t H) {
     return (K >= 4 ? size_t(RPC) * Wmax * 8 : 0) + size_t(RPC) * 8 + size_t(RPC + 1) * 4 + size_t(Dmax) * 4 +
-           size_t(H) * 6 + 16;
+           size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
 }
 
 static int set_lds_attrs() {
@@ -554,8 +572,10 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         uint32_t per_cu = std::max<uint32_t>(1, uint32_t(R.lds_per_block / (lds + kStaticLdsReserve)));
         per_cu = std::min<uint32_t>(per_cu, 2048 / block);
         const uint32_t grid = std::min<uint32_t>(nt, uint32_t(R.num_cus) * per_cu);
+        uint32_t FW, fshift;
+        filter_geometry(Dmax, &FW, &fshift);
         hipLaunchKernelGGL(clique_block_kernel<K>, dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt, cur,
-                           x.d_result, Dmax, W, H, hshift);
+                           x.d_result, Dmax, W, H, hshift, FW, fshift);
         return KZ_OK;
     }
     // hubs: rows kernel, then (k >= 4) the count kernel on the same stream
@@ -571,7 +591,8 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     const uint32_t grid = std::min<uint32_t>(hp.nitems, uint32_t(R.num_cus) * per_cu);
     const HubMeta* meta = static_cast<const HubMeta*>(x.hub_meta[K]);
     hipLaunchKernelGGL(hub_rows_kernel<K>, dim3(grid), dim3(block), lds, s, qrefs, meta, hp.nhubs, x.hub_items[K],
-                       hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax, hp.wmax, hp.rpc, H);
+                       hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax, hp.wmax, hp.rpc, H,
+                       filter_words(hp.dmax));
     if (K >= 4) {
         const uint32_t cgrid = std::min<uint32_t>((hp.nrows + 3) / 4, uint32_t(R.num_cus) * 8);
         hipLaunchKernelGGL(hub_count_kernel<K>, dim3(cgrid), dim3(256), 0, s, meta, hp.nhubs, x.hub_rows[K], hp.nrows,
@@ -717,17 +738,20 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
                        int(std::max<size_t>(1, queries.size())));
     KZ_HIP(hipEventRecord(fork, main));
     int e = KZ_OK;
+    // serial: one chain on the main stream; otherwise one branch per group
     for (size_t i = 0; i < x.groups.size() && e == KZ_OK; ++i) {
-        KZ_HIP(hipStreamWaitEvent(br[i], fork, 0));
+        hipStream_t s = cfg.serial ? main : br[i];
+        if (!cfg.serial) KZ_HIP(hipStreamWaitEvent(s, fork, 0));
         const CliqueJob::Group& gr = x.groups[i];
-        if (gr.k == 3) e = launch_group<3>(x, gr, br[i]);
-        else if (gr.k == 4) e = launch_group<4>(x, gr, br[i]);
-        else e = launch_group<5>(x, gr, br[i]);
+        if (gr.k == 3) e = launch_group<3>(x, gr, s);
+        else if (gr.k == 4) e = launch_group<4>(x, gr, s);
+        else e = launch_group<5>(x, gr, s);
     }
-    for (size_t i = 0; i < x.groups.size(); ++i) {
-        KZ_HIP(hipEventRecord(join[i], br[i]));
-        KZ_HIP(hipStreamWaitEvent(main, join[i], 0));
-    }
+    if (!cfg.serial)
+        for (size_t i = 0; i < x.groups.size(); ++i) {
+            KZ_HIP(hipEventRecord(join[i], br[i]));
+            KZ_HIP(hipStreamWaitEvent(main, join[i], 0));
+        }
     hipGraph_t graph;
     KZ_HIP(hipStreamEndCapture(main, &graph));
     if (e == KZ_OK && hipGetLastError() != hipSuccess) {
@@ -773,16 +797,15 @@ int clique_job_run(CliqueJob& x, uint64_t* counts, double* device_ms) {
 // Candidate bucket configurations tried by the autotuner.
 std::vector<BucketCfg> tuning_candidates() {
     std::vector<BucketCfg> out;
-    const uint32_t w0s[] = {32, 64};
-    const uint32_t b1s[] = {128, 256};
-    const uint32_t blks[] = {256, 512};
-    for (uint32_t w0 : w0s)
-        for (uint32_t b1 : b1s)
-            for (uint32_t bk : blks) {
+    const uint32_t bounds[][4] = {{64, 128, 512, 1024}, {64, 192, 384, 1024}, {32, 256, 512, 1024}, {64, 256, 512, 1024}};
+    const uint32_t blocks[][kNumBuckets] = {{256, 256, 256, 512, 512}, {256, 256, 512, 1024, 1024}};
+    for (const auto& bd : bounds)
+        for (const auto& bk : blocks)
+            for (uint32_t serial = 0; serial < 2; ++serial) {
                 BucketCfg c = default_buckets();
-                c.hi[0] = w0;
-                c.hi[1] = b1;
-                c.block[1] = c.block[2] = bk;
+                for (int b = 0; b < 4; ++b) c.hi[b] = bd[b];
+                for (int b = 0; b < kNumBuckets; ++b) c.block[b] = bk[b];
+                c.serial = serial;
                 out.push_back(c);
             }
     return out;

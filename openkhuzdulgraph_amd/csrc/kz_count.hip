@@ -82,6 +82,7 @@ static BucketCfg graph_buckets(const DeviceGraph& d, int k) {
         c.hi[b] = d.bucket_hi[k][b];
         c.block[b] = d.bucket_block[k][b];
     }
+    c.serial = d.bucket_serial[k];
     return c;
 }
 
@@ -90,6 +91,7 @@ static void store_buckets(DeviceGraph& d, int k, const BucketCfg& c) {
         d.bucket_hi[k][b] = c.hi[b];
         d.bucket_block[k][b] = c.block[b];
     }
+    d.bucket_serial[k] = c.serial;
     d.bucket_set[k] = true;
     d.clique_exec[k].reset();
 }
@@ -98,12 +100,14 @@ static BucketCfg from_abi(const kz_bucket_cfg& a) {
     BucketCfg c = default_buckets();
     for (int b = 0; b < 4; ++b) c.hi[b] = a.bound[b];
     for (int b = 0; b < 5; ++b) c.block[b] = a.block[b];
+    c.serial = a.serial ? 1u : 0u;
     return c;
 }
 
 static void to_abi(const BucketCfg& c, kz_bucket_cfg* a) {
     for (int b = 0; b < 4; ++b) a->bound[b] = c.hi[b];
     for (int b = 0; b < 5; ++b) a->block[b] = c.block[b];
+    a->serial = c.serial;
 }
 
 static std::shared_ptr<void> wrap_job(CliqueJob* j) {

@@ -66,6 +66,7 @@ struct DeviceGraph {
     bool bucket_set[6] = {};
     uint32_t bucket_hi[6][5] = {};
     uint32_t bucket_block[6][5] = {};
+    uint32_t bucket_serial[6] = {};
 };
 
 // Device scratch arena that grows on demand and is reused across calls.

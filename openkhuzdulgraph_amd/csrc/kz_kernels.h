@@ -154,10 +154,12 @@ struct CliqueView {
     DeviceGraph* owner = nullptr;
 };
 
-// Degree-bucket bounds on |N+(u)| (last bucket unbounded) and block sizes.
+// Degree-bucket bounds on |N+(u)| (last bucket unbounded), block sizes and
+// the launch policy (serial: bucket kernels one after another).
 struct BucketCfg {
     uint32_t hi[kNumBuckets];
     uint32_t block[kNumBuckets];
+    uint32_t serial;
 };
 BucketCfg default_buckets();
 int check_buckets(const BucketCfg& c);

@@ -102,30 +102,88 @@ __device__ __forceinline__ bool row_desc(const uint64_t* rowptr, uint32_t v, uin
     return false;
 }
 
+// The root's list in LDS: an open-addressing hash (id -> index) and a
+// one-hash membership filter of 2^(32 - fshift) bits (at least 16 per key)
+// that rejects most non-members with a single LDS read before the probe.
+struct RootSet {
+    uint32_t* hk;
+    uint16_t* hv;
+    uint32_t hmask, hshift;
+    uint32_t* fb;
+    uint32_t fshift;
+};
+
+__device__ __forceinline__ uint32_t filter_bit(uint32_t x, uint32_t fshift) { return (x * 0x85EBCA6Bu) >> fshift; }
+
+// Clears the hash (H slots) and filter (fw words) over threads tid..T.
+__device__ __forceinline__ void root_clear(const RootSet& s, uint32_t H, uint32_t fw, uint32_t tid, uint32_t T) {
+    for (uint32_t i = tid; i < H; i += T) s.hk[i] = kEmpty;
+    for (uint32_t i = tid; i < fw; i += T) s.fb[i] = 0;
+}
+
+__device__ __forceinline__ void root_insert(const RootSet& s, uint32_t x, uint16_t idx) {
+    hash_insert(s.hk, s.hv, s.hmask, s.hshift, x, idx);
+    const uint32_t b = filter_bit(x, s.fshift);
+    atomicOr(&s.fb[b >> 5], 1u << (b & 31));
+}
+
+__device__ __forceinline__ uint32_t root_find(const RootSet& s, uint32_t x) {
+    const uint32_t b = filter_bit(x, s.fshift);
+    if (!((s.fb[b >> 5] >> (b & 31)) & 1u)) return kNotFound;
+    return hash_find(s.hk, s.hv, s.hmask, s.hshift, x);
+}
+
+// Filter geometry for lists of up to dmax ids: words and shift.
+__host__ __device__ inline void filter_geometry(uint32_t dmax, uint32_t* words, uint32_t* fshift) {
+    uint32_t bits = 6;
+    while ((1u << bits) < 16u * dmax && bits < 31) ++bits;
+    *words = (1u << bits) >> 5;
+    *fshift = 32 - bits;
+}
+
+// Largest i in [lo, hi) with p[i] <= f, given p[lo] <= f.
+__device__ __forceinline__ uint32_t row_in(const uint32_t* p, uint32_t lo, uint32_t hi, uint32_t f) {
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p[mid] <= f) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Streams groups [0, G) of rows [0, nr) described by desc and gp (exclusive
-// prefix of the group counts, nr + 1 entries) over threads tid = 0 .. T-1.
-// Every entry x <= lmax found in the hash of the root's list at index j is
-// a hit: hit(r, j, p) with p the entry's position in col.  Each lane issues
-// kGroups independent 16-byte loads before it probes.
+// prefix of the group counts, nr + 1 entries) over threads tid = 0 .. T-1
+// (whole waves).  Every entry x <= lmax found in the root's set at index j
+// is a hit: hit(r, j, p) with p the entry's position in col.  Each wave
+// takes 64 * kGroups consecutive groups per iteration: two wave-uniform
+// searches bound their rows, each lane searches only that window, and each
+// lane issues kGroups independent 16-byte loads (consecutive across the
+// wave) before it probes.
 template <class Hit>
 __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigned long long* desc, const uint32_t* gp,
-                                              uint32_t nr, uint32_t G, uint32_t tid, uint32_t T, const uint32_t* hk,
-                                              const uint16_t* hv, uint32_t hmask, uint32_t hshift, uint32_t lmax,
-                                              Hit&& hit) {
-    for (uint32_t base = tid; base < G; base += kGroups * T) {
+                                              uint32_t nr, uint32_t G, uint32_t tid, uint32_t T, const RootSet& rs,
+                                              uint32_t lmax, Hit&& hit) {
+    const uint32_t lane = tid & 63;
+    const uint32_t wofs = (tid - lane) * kGroups;
+    for (uint32_t cbase = 0; cbase < G; cbase += kGroups * T) {
+        const uint32_t g0 = cbase + wofs;
+        if (g0 >= G) continue;
+        const uint32_t g1 = (g0 + 64 * kGroups < G ? g0 + 64 * kGroups : G) - 1;
+        const uint32_t rlo = row_of(gp, nr, g0);
+        const uint32_t rhi = row_in(gp, rlo, nr, g1) + 1;
         u32x4 y[kGroups];
         uint32_t rr[kGroups], lo[kGroups], hi[kGroups];
         uint64_t at[kGroups];
 #pragma unroll
         for (int k = 0; k < kGroups; ++k) {
-            const uint32_t g = base + uint32_t(k) * T;
+            const uint32_t g = g0 + uint32_t(k) * 64 + lane;
             y[k] = u32x4{0u, 0u, 0u, 0u};
             rr[k] = 0;
             lo[k] = 4;
             hi[k] = 0;
             at[k] = 0;
-            if (g < G) {
-                const uint32_t r = row_of(gp, nr, g);
+            if (g <= g1) {
+                const uint32_t r = row_in(gp, rlo, rhi, g);
                 const uint32_t q = g - gp[r];
                 const uint64_t ds = desc[r];
                 const uint64_t vb = ds & kStartMask;
@@ -143,7 +201,7 @@ __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigne
             for (int c = 0; c < 4; ++c) {
                 const uint32_t x = y[k][c];
                 if (uint32_t(c) >= lo[k] && uint32_t(c) < hi[k] && x <= lmax) {
-                    const uint32_t j = hash_find(hk, hv, hmask, hshift, x);
+                    const uint32_t j = root_find(rs, x);
                     if (j != kNotFound) hit(rr[k], j, at[k] + uint32_t(c));
                 }
             }

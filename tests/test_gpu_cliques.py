@@ -129,4 +129,10 @@ def test_hub_and_bucket_paths_exact(kzdev, k):
         assert dg.count_cliques(k, "persistent") == want, (name, "hub-heavy")
         dg.set_buckets(k, [8, 24, 40, 56], [256, 512, 512, 512, 256])
         assert dg.count_cliques(k, "persistent") == want, (name, "hub-heavy-256")
+        # both launch policies and 1024-thread workgroups on every bucket
+        for serial in (False, True):
+            dg.set_buckets(k, [64, 256, 512, 1024], [256, 1024, 1024, 1024, 1024], serial)
+            assert dg.count_cliques(k, "persistent") == want, (name, "1024", serial)
+            dg.set_buckets(k, [32, 96, 160, 224], [256, 256, 512, 1024, 512], serial)
+            assert dg.count_cliques(k, "persistent") == want, (name, "mixed", serial)
         dg.release()
