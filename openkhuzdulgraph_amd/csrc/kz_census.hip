@@ -155,8 +155,6 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
     p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
     __shared__ uint32_t sTask, sTmp[T / 64];
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
     const RootSet rs{hk, hv, H - 1, hshift, fb, fshift};
     while (true) {
         if (threadIdx.x == 0) sTask = atomicAdd(cursor, 1u);
@@ -175,10 +173,10 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
         for (uint32_t i = threadIdx.x; i < d; i += T) root_insert(rs, L[i], uint16_t(i));
         const uint32_t nr = d - 1;
         for (uint32_t r = threadIdx.x; r <= nr; r += T) {
-            uint32_t groups = 0;
+            uint32_t groups = 0, len = 0;
             uint64_t desc = 0;
             if (r < nr) {
-                row_desc(frowptr, L[r], d - 1 - r, &desc, &groups);
+                row_desc(frowptr, L[r], d - 1 - r, &desc, &groups, &len);
                 rb[r] = desc;
             }
             rp[r] = groups;
@@ -191,17 +189,9 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
             atomicAdd(&tmid[at], 1u);
         };
         stream_groups(fcol, rb, rp, nr, G, threadIdx.x, T, rs, L[d - 1], hit);
-        // skewed rows: one wave per row binary-searches the candidates
-        for (uint32_t r = wave; r < nr; r += T / 64) {
-            const uint64_t ds = rb[r];
-            if (!(ds & kSkewBit)) continue;
-            const uint64_t b = ds & kStartMask;
-            const uint32_t nv = uint32_t(ldg(frowptr, L[r] + 1) - b);
-            for (uint32_t j = r + 1 + lane; j < d; j += 64) {
-                const uint32_t pos = find_global(fcol + b, nv, L[j]);
-                if (pos < nv) hit(r, j, b + pos);
-            }
-        }
+        // skewed rows: each wave searches the candidates of its own rows
+        search_skewed(frowptr, fcol, rb, L, d, 0, nr, threadIdx.x >> 6, T / 64, hit,
+                      [&](uint32_t r) { return uint32_t(ldg(frowptr, L[r] + 1) - (rb[r] & kStartMask)); });
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < d; i += T) troot[ub + i] = cnt[i];
         __syncthreads();

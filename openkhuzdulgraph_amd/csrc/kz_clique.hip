@@ -63,7 +63,9 @@ struct HubMeta {
 
 // ------------------------------------------------------------ wave tasks --
 // One task per wave, |N+(u)| <= 64.  LDS per wave: L[64], 128-slot hash,
-// 64 bitmap rows (one u64 each) and the row descriptors.
+// 64 bitmap rows (one u64 each) and the row descriptors.  Tasks are small,
+// so the chain cursor -> root -> list bounds of the NEXT task is issued
+// while the current one streams and counts.
 template <int K>
 __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __restrict__ queries,
                                                           const uint32_t* __restrict__ task_v,
@@ -90,24 +92,31 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
     uint32_t* rp = sRp[w];
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
+    auto fetch = [&]() {
+        uint32_t v = 0;
+        if (lane == 0) v = atomicAdd(cursor, 1u);
+        return __shfl(v, 0);
+    };
+    // current task: index, query, list start and length, graph arrays
+    uint32_t t = fetch(), q = 0, d = 0;
+    uint64_t ub = 0;
     const uint64_t* rowptr = nullptr;
     const uint32_t* col = nullptr;
-    while (true) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(cursor, 1u);
-        t = __shfl(t, 0);
-        if (t >= ntasks) break;
-        const uint32_t q = task_q ? task_q[t] : 0u;
+    if (t < ntasks) {
+        q = task_q ? task_q[t] : 0u;
+        rowptr = queries[q].rowptr;
+        col = queries[q].col;
+        const uint32_t u = task_v[t];
+        ub = ldg(rowptr, u);
+        d = uint32_t(ldg(rowptr, u + 1) - ub);
+    }
+    while (t < ntasks) {
         if (q != qcur) {
             if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
             qcur = q;
-            rowptr = queries[q].rowptr;
-            col = queries[q].col;
         }
-        const uint32_t u = task_v[t];
-        const uint64_t ub = ldg(rowptr, u);
-        const uint32_t d = uint32_t(ldg(rowptr, u + 1) - ub);
         const uint32_t x = lane < d ? ldg(col, ub + lane) : 0xFFFFFFFFu;
+        const uint32_t tn = fetch();
         L[lane] = x;
         hk[lane] = kEmpty;
         hk[lane + 64] = kEmpty;
@@ -115,17 +124,22 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
         rows[lane] = 0;
         // row descriptors: lane r owns row r
         // (rows >= d - 1 are empty, so rp[d - 1] is the group total)
-        uint32_t groups = 0;
+        uint32_t groups = 0, len = 0;
         uint64_t desc = 0;
         bool skewed = false;
-        if (lane + 1 < d) skewed = row_desc(rowptr, x, d - 1 - lane, &desc, &groups);
+        if (lane + 1 < d) skewed = row_desc(rowptr, x, d - 1 - lane, &desc, &groups, &len);
         rb[lane] = desc;
         uint32_t excl;
         const uint32_t G = wave_exclusive_scan(groups, &excl);
         rp[lane] = excl;
-        const uint64_t skmask = __ballot(skewed);
         wave_lds_sync();
         if (lane < d) root_insert(rs, x, uint16_t(lane));
+        // next task's query and root (the cursor atomic has returned by now)
+        uint32_t qn = q, un = 0;
+        if (tn < ntasks) {
+            qn = task_q ? task_q[tn] : 0u;
+            un = task_v[tn];
+        }
         wave_lds_sync();
         const uint32_t lmax = L[d - 1];
         stream_groups(col, rb, rp, d - 1, G, lane, 64, rs, lmax,
@@ -133,19 +147,29 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
                           if (K == 3) acc += 1;
                           else atomicOr(&rows[r], 1ull << j);
                       });
-        // skewed rows: binary-search the remaining candidates in N+(v)
-        unsigned long long sm = skmask;
-        while (sm) {
-            const uint32_t i = uint32_t(__builtin_ctzll(sm));
-            sm &= sm - 1;
-            const uint64_t b = rb[i] & kStartMask;
-            const uint32_t n = uint32_t(ldg(rowptr, L[i] + 1) - b);
-            const uint32_t j = i + 1 + lane;
-            if (j < d && bsearch_global(col + b, n, L[j])) {
-                if (K == 3) acc += 1;
-                else atomicOr(&rows[i], 1ull << j);
+        // next task's list bounds
+        const uint64_t* rowptr_n = rowptr;
+        const uint32_t* col_n = col;
+        uint64_t ubn = 0;
+        uint32_t dn = 0;
+        if (tn < ntasks) {
+            if (qn != q) {
+                rowptr_n = queries[qn].rowptr;
+                col_n = queries[qn].col;
             }
+            ubn = ldg(rowptr_n, un);
+            dn = uint32_t(ldg(rowptr_n, un + 1) - ubn);
         }
+        // skewed rows: their (row, candidate) pairs spread over the lanes;
+        // lane r holds row r's list length
+        if (__ballot(skewed))
+            search_skewed(
+                rowptr, col, rb, L, d, 0, d - 1, 0, 1,
+                [&](uint32_t r, uint32_t j, uint64_t) {
+                    if (K == 3) acc += 1;
+                    else atomicOr(&rows[r], 1ull << j);
+                },
+                [&](uint32_t) { return len; });
         wave_lds_sync();
         if (K >= 4 && lane < d) {
             const unsigned long long ri = rows[lane];
@@ -167,6 +191,12 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
             }
         }
         wave_lds_sync();
+        t = tn;
+        q = qn;
+        rowptr = rowptr_n;
+        col = col_n;
+        ub = ubn;
+        d = dn;
     }
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
@@ -225,15 +255,12 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
                                                  const uint32_t* L, uint32_t d, const RootSet& rs, uint32_t r0, uint32_t r1,
                                                  unsigned long long* rows_out, uint32_t ws, unsigned long long* rb,
                                                  uint32_t* rp, uint32_t* tmp, unsigned long long& acc) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t nwaves = blockDim.x >> 6;
     const uint32_t nr = r1 - r0;
     for (uint32_t r = threadIdx.x; r <= nr; r += blockDim.x) {
         const uint32_t i = r0 + r;
-        uint32_t groups = 0;
+        uint32_t groups = 0, len = 0;
         uint64_t desc = 0;
-        if (r < nr && i + 1 < d) row_desc(rowptr, L[i], d - 1 - i, &desc, &groups);
+        if (r < nr && i + 1 < d) row_desc(rowptr, L[i], d - 1 - i, &desc, &groups, &len);
         if (r < nr) rb[r] = desc;
         rp[r] = groups;
     }
@@ -244,20 +271,13 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
                       if (K == 3) acc += 1;
                       else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
                   });
-    // skewed rows: one wave per row binary-searches the candidates in N+(v)
-    for (uint32_t r = wave; r < nr; r += nwaves) {
-        const uint64_t ds = rb[r];
-        if (!(ds & kSkewBit)) continue;
-        const uint32_t i = r0 + r;
-        const uint64_t b = ds & kStartMask;
-        const uint32_t n = uint32_t(ldg(rowptr, L[i] + 1) - b);
-        for (uint32_t j = i + 1 + lane; j < d; j += 64) {
-            if (bsearch_global(col + b, n, L[j])) {
-                if (K == 3) acc += 1;
-                else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
-            }
-        }
-    }
+    // skewed rows: each wave searches the candidates of its own rows
+    search_skewed(rowptr, col, rb, L, d, r0, nr, threadIdx.x >> 6, blockDim.x >> 6,
+                  [&](uint32_t r, uint32_t j, uint64_t) {
+                      if (K == 3) acc += 1;
+                      else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
+                  },
+                  [&](uint32_t r) { return uint32_t(ldg(rowptr, L[r0 + r] + 1) - (rb[r] & kStartMask)); });
     __syncthreads();
 }
 

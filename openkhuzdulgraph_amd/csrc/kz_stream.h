@@ -82,17 +82,20 @@ constexpr int kGroups = 4;
 // "skewed" row and bits 62..63 hold ve & 3.  A streamed row is read as
 // aligned 16-byte groups from vb & ~3 (*groups of them); the first group
 // starts at entry vb & 3 and the last one ends at ve & 3 (0: full).  A
-// skewed row, whose list is much longer than its candidates, is not
-// streamed (*groups = 0): its candidates are binary-searched in the list.
-// Returns whether the row is skewed.  Column arrays are allocated with
-// col_alloc_bytes, so the last group stays inside the allocation.
+// skewed row, whose list is more than kSkewFactor times longer than its
+// candidates, is not streamed (*groups = 0): its candidates are
+// binary-searched in the list.  *len gets ve - vb.  Returns whether the row
+// is skewed.  Column arrays are allocated with col_alloc_bytes, so the last
+// group stays inside the allocation.
 constexpr uint64_t kSkewBit = 1ull << 61;
 constexpr uint64_t kStartMask = kSkewBit - 1;
+constexpr uint64_t kSkewFactor = 4;
 
 __device__ __forceinline__ bool row_desc(const uint64_t* rowptr, uint32_t v, uint32_t cand, uint64_t* desc,
-                                         uint32_t* groups) {
+                                         uint32_t* groups, uint32_t* len) {
     const uint64_t vb = ldg(rowptr, v), ve = ldg(rowptr, v + 1);
-    if (ve - vb > 16ull * cand) {
+    *len = uint32_t(ve - vb);
+    if (ve - vb > kSkewFactor * cand) {
         *desc = vb | kSkewBit;
         *groups = 0;
         return true;
@@ -213,6 +216,56 @@ __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigne
 __device__ __forceinline__ uint32_t find_global(const uint32_t* __restrict__ a, uint32_t n, uint32_t x) {
     const uint32_t i = lower_bound_global(a, n, x);
     return (i < n && ldg(a, i) == x) ? i : n;
+}
+
+// Candidates of the skewed rows among rows [0, nr) (list indices r0 + r of
+// L, d ids; candidates of row r are L[r0 + r + 1 .. d)), binary-searched in
+// the rows' lists.  Wave `wave` of nw takes rows r = wave, wave + nw, ...,
+// 64 at a time, and spreads their (row, candidate) pairs over its lanes, so
+// the searches of 64 pairs are in flight together; no workgroup barrier, so
+// it overlaps other waves' streaming.  hit(r, j, p): candidate j found at
+// position p of col.  row_len(r), called by the lane that takes row r, gives
+// the row's list length.  Call with every lane of the wave.
+template <class Hit, class Len>
+__device__ __forceinline__ void search_skewed(const uint64_t* rowptr, const uint32_t* col,
+                                              const unsigned long long* desc, const uint32_t* L, uint32_t d,
+                                              uint32_t r0, uint32_t nr, uint32_t wave, uint32_t nw, Hit&& hit,
+                                              Len&& row_len) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t rbase = wave; rbase < nr; rbase += 64 * nw) {
+        const uint32_t r = rbase + lane * nw;
+        uint32_t cnt = 0, n = 0;
+        uint64_t b = 0;
+        if (r < nr) {
+            const uint64_t ds = desc[r];
+            if (ds & kSkewBit) {
+                b = ds & kStartMask;
+                cnt = d - 1 - (r0 + r);
+                n = row_len(r);
+            }
+        }
+        uint32_t excl;
+        const uint32_t P = wave_exclusive_scan(cnt, &excl);
+        for (uint32_t pbase = 0; pbase < P; pbase += 64) {
+            const uint32_t p = pbase + lane;
+            // owning lane: the largest l with excl(l) <= p
+            uint32_t l = 0;
+#pragma unroll
+            for (uint32_t step = 32; step; step >>= 1) {
+                const uint32_t e = __shfl(excl, int(l + step));
+                if (e <= p) l += step;
+            }
+            const uint32_t rr = __shfl(r, int(l));
+            const uint64_t bb = __shfl(b, int(l));
+            const uint32_t nn = __shfl(n, int(l));
+            const uint32_t e0 = __shfl(excl, int(l));
+            if (p < P) {
+                const uint32_t j = r0 + rr + 1 + (p - e0);
+                const uint32_t pos = find_global(col + bb, nn, L[j]);
+                if (pos < nn) hit(rr, j, bb + pos);
+            }
+        }
+    }
 }
 
 }  // namespace kz
