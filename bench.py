@@ -14,6 +14,17 @@ kautomine four_clique schedule on a bounded sample of roots, on this host's
 cores.
 
     python bench.py [--steps K] [--warmup W] [--scale 22] [--plain-steps 1]
+
+Other workloads (each prints its own JSON line; the default is the one above):
+    --workload census   induced 3/4-vertex census of a seeded Chung-Lu graph
+                        (n 4.85M, 43M draws, exponent 2.3, degree cap 20000)
+    --workload batch    many seeded ER/RMAT graphs (seed 1000+i, n ~ U[2^10,
+                        2^18], average degree ~ U[4, 64]), triangle, 4- and
+                        5-clique of every graph in one hipGraph launch
+    --workload serve    a seeded Poisson stream of queries through kz_server:
+                        triangle and 4-clique on a large RMAT graph, 5-clique
+                        and motif census on a pool of batch-workload graphs,
+                        at several arrival rates; p50/p99 latency
 """
 
 from __future__ import annotations
@@ -71,6 +82,198 @@ def cpu_baseline(g, budget_s: float, threads: int) -> dict:
     }
 
 
+def pool_graph(kz, i: int):
+    """Graph i of the batch workload (seed 1000 + i): even i Erdos-Renyi,
+    odd i RMAT (.57/.19/.19/.05); n ~ U[2^10, 2^18], average degree ~ U[4, 64]
+    (RMAT: scale round(log2 n), edge factor round(degree / 2))."""
+    import math
+
+    import numpy as np
+
+    rng = np.random.default_rng(1000 + i)
+    n = int(rng.integers(2**10, 2**18 + 1))
+    deg = float(rng.uniform(4, 64))
+    if i % 2 == 0:
+        return kz.gen_erdos_renyi(n, int(n * deg / 2), 1000 + i)
+    return kz.gen_rmat(max(10, round(math.log2(n))), max(2, round(deg / 2)), 0.57, 0.19, 0.19, 1000 + i)
+
+
+def base_line(metric: str, value, unit: str, steps: int, warmup: int, ms: float, higher: bool, config: dict) -> dict:
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": "uint32", "data": "synthetic", "config": config}
+
+
+def bench_census(kz, args) -> int:
+    t0 = time.perf_counter()
+    g = kz.gen_chung_lu(4850000, 43000000, 2.3, 20000, 1)
+    log(f"generated {g} in {time.perf_counter() - t0:.1f} s")
+    kz.init(0)
+    dg = g.to_device()
+    t0 = time.perf_counter()
+    first = dg.motif_census()
+    log(f"first call (GPU relabelling + census) {time.perf_counter() - t0:.2f} s")
+    for _ in range(args.warmup):
+        dg.motif_census()
+    ms = []
+    for _ in range(args.steps):
+        c = dg.motif_census()
+        if c != first:
+            raise SystemExit("census changed between steps")
+        ms.append(dg.last_stats["device_ms"])
+    avg = sum(ms) / len(ms)
+    check = None
+    if args.check:
+        import oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        want = oracle.motif_census(g.rowptr, g.col, threads=threads)
+        check = {"equal": want == first, "oracle_seconds": time.perf_counter() - t0, "threads": threads}
+        log(f"oracle check: {check}")
+        if want != first:
+            raise SystemExit("census differs from the oracle")
+    from openkhuzdulgraph_amd._ffi import CENSUS_PATTERNS
+    line = base_line("induced 3/4-vertex motif census: s, 1 MI355X", avg / 1e3, "s", args.steps, args.warmup, avg,
+                     False, {"workload": "chunglu_4.85M_43Mdraws_census", "vertices": g.num_vertices,
+                             "edges": g.num_entries // 2, "max_degree": g.max_degree, "seed": 1})
+    line["counts"] = dict(zip(CENSUS_PATTERNS, first))
+    line["oracle_check"] = check
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_batch(kz, args) -> int:
+    t0 = time.perf_counter()
+    graphs = [pool_graph(kz, i).oriented() for i in range(args.graphs)]
+    ent = sum(g.num_entries for g in graphs)
+    log(f"generated {args.graphs} graphs ({ent} oriented entries) in {time.perf_counter() - t0:.1f} s")
+    kz.init(0)
+    dgs = [g.to_device() for g in graphs]
+    ks = [3, 4, 5]
+    b = kz.Batch(dgs, ks)
+    t0 = time.perf_counter()
+    first = b.run()
+    prep = time.perf_counter() - t0
+    log(f"first launch (task lists, capture) {prep:.2f} s")
+    for _ in range(args.warmup):
+        b.run()
+    ms = []
+    for _ in range(args.steps):
+        if not (b.run() == first).all():
+            raise SystemExit("batch counts changed between launches")
+        ms.append(b.last_stats["device_ms"])
+    avg = sum(ms) / len(ms)
+    check = None
+    if args.check:
+        import oracle
+
+        idx = list(range(0, args.graphs, max(1, args.graphs // 16)))
+        for i in idx:
+            for j, k in enumerate(ks):
+                if dgs[i].count_cliques(k, "persistent") != int(first[i, j]):
+                    raise SystemExit(f"batch differs from the single-graph run: graph {i} k {k}")
+        small = sorted(range(args.graphs), key=lambda i: graphs[i].num_entries)[:4]
+        apps = {3: "kautomine_triangle", 4: "kautomine_four_clique", 5: "kautomine_five_clique"}
+        for i in small:
+            for j, k in enumerate(ks):
+                if oracle.run_app(apps[k], graphs[i].rowptr, graphs[i].col)[1][0] != int(first[i, j]):
+                    raise SystemExit(f"batch differs from the oracle: graph {i} k {k}")
+        check = {"single_runs": len(idx), "oracle_graphs": len(small)}
+        log(f"checks: {check}")
+    q = args.graphs * len(ks)
+    line = base_line("batched exact clique counts: queries/s, 1 MI355X", q / (avg / 1e3), "queries/s", args.steps,
+                     args.warmup, avg, True, {"workload": f"batch_{args.graphs}_graphs_k345", "graphs": args.graphs,
+                                              "oriented_entries": ent, "queries": q})
+    line["prepare_seconds"] = prep
+    line["totals"] = {f"k{k}": int(first[:, j].sum()) for j, k in enumerate(ks)}
+    line["check"] = check
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_serve(kz, args) -> int:
+    import numpy as np
+
+    t0 = time.perf_counter()
+    big = kz.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed).oriented()
+    pool_sym = [pool_graph(kz, i) for i in range(args.graphs)]
+    pool_ori = [g.oriented() for g in pool_sym]
+    log(f"generated {big} and {args.graphs} pool graphs in {time.perf_counter() - t0:.1f} s")
+    kz.init(0)
+    dbig = big.to_device()
+    dsym = [g.to_device() for g in pool_sym]
+    dori = [g.to_device() for g in pool_ori]
+    # query kinds: (graph, kind, k); large graph: triangle, 4-clique; pool:
+    # 5-clique (oriented copy) and the motif census (symmetric copy)
+    def kinds(i):
+        return [(dbig, "clique", 3, "big"), (dbig, "clique", 4, "big"), (dori[i], "clique", 5, f"pool{i}"),
+                (dsym[i], "census", 0, f"pool{i}")]
+
+    rng = np.random.default_rng(2)
+    budget = int(args.budget_gb * (1 << 30))
+    # each query kind alone first: the reference result for the mix
+    alone = {}
+    with kz.Server(hbm_budget=budget, max_inflight=1) as srv:
+        for i in range(args.graphs):
+            for g, kind, k, name in kinds(i):
+                key = (name, kind, k)
+                if key in alone:
+                    continue
+                r = srv.poll(srv.submit(g, kind, k or 4), wait=True)
+                if r["status"] != 0:
+                    raise SystemExit(f"query {key} failed: {r}")
+                alone[key] = r["counts"]
+    log(f"alone results for {len(alone)} query kinds")
+    rates = [float(x) for x in args.rates.split(",")]
+    per_rate = []
+    with kz.Server(hbm_budget=budget, max_inflight=8) as srv:
+        for rate in rates:
+            t_start = time.perf_counter()
+            next_t = t_start
+            outstanding, lat, done, mismatches = {}, [], 0, 0
+            while True:
+                now = time.perf_counter()
+                if now >= next_t and now - t_start < args.seconds:
+                    i = int(rng.integers(0, args.graphs))
+                    g, kind, k, name = kinds(i)[int(rng.integers(0, 4))]
+                    outstanding[srv.submit(g, kind, k or 4)] = (name, kind, k)
+                    next_t += float(rng.exponential(1.0 / rate))
+                    continue
+                for t in list(outstanding):
+                    r = srv.poll(t)
+                    if r is None:
+                        continue
+                    key = outstanding.pop(t)
+                    if r["status"] != 0 or r["counts"] != alone[key]:
+                        mismatches += 1
+                    lat.append(r["total_ms"])
+                    done += 1
+                if now - t_start >= args.seconds and not outstanding:
+                    break
+                time.sleep(0.0002)
+            wall = time.perf_counter() - t_start
+            lat.sort()
+            entry = {"rate": rate, "queries": done, "queries_per_s": done / wall,
+                     "p50_ms": lat[len(lat) // 2] if lat else None,
+                     "p99_ms": lat[min(len(lat) - 1, int(0.99 * len(lat)))] if lat else None,
+                     "mismatches": mismatches}
+            per_rate.append(entry)
+            log(f"rate {rate}/s: {entry}")
+            if mismatches:
+                raise SystemExit(f"{mismatches} queries differ from their alone results")
+    best = max(per_rate, key=lambda e: e["queries_per_s"])
+    line = base_line("serving exact counts: queries/s and p50/p99 latency, 1 MI355X", best["queries_per_s"],
+                     "queries/s", len(rates), 0, best["p50_ms"], True,
+                     {"workload": f"serve_rmat{args.scale}_ef{args.edge_factor}_pool{args.graphs}",
+                      "large_graph_vertices": big.num_vertices, "large_graph_oriented_edges": big.num_entries,
+                      "pool_graphs": args.graphs, "budget_bytes": budget, "max_inflight": 8,
+                      "mix": "large: triangle, 4-clique; pool: 5-clique, motif census (uniform)"})
+    line["per_rate"] = per_rate
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +289,12 @@ def main() -> int:
     ap.add_argument("--no-dgraph", action="store_true", help="skip the .dgraph round trip")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
+    ap.add_argument("--workload", choices=["clique", "census", "batch", "serve"], default="clique")
+    ap.add_argument("--graphs", type=int, default=256, help="batch/serve: number of pool graphs")
+    ap.add_argument("--check", action="store_true", help="census/batch: also check against the oracle")
+    ap.add_argument("--rates", default="4,8,16,32", help="serve: arrival rates (queries/s)")
+    ap.add_argument("--seconds", type=float, default=8.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="serve: HBM budget for prepared executions")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -103,6 +312,13 @@ def main() -> int:
         return 2
 
     import openkhuzdulgraph_amd as kz
+
+    if args.workload == "census":
+        return bench_census(kz, args)
+    if args.workload == "batch":
+        return bench_batch(kz, args)
+    if args.workload == "serve":
+        return bench_serve(kz, args)
 
     t0 = time.perf_counter()
     sym = kz.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)

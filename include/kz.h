@@ -222,6 +222,46 @@ int kz_clique_workload(kz_graph g, uint64_t* out6);
 #define KZ_CENSUS_PATTERNS 8
 int kz_motif_census(kz_graph g, uint64_t* counts, kz_run_stats* stats);
 
+/* ---- Serving --------------------------------------------------------------
+ * A server runs queries on resident graphs from a worker thread.  Queries
+ * are admitted in arrival order while the device memory of the prepared
+ * executions the server keeps (root task lists, bitmap slabs, census
+ * copies) stays within hbm_budget bytes; idle prepared executions are
+ * released, least recently used first, to make room, and a query whose own
+ * execution exceeds the budget fails with KZ_ERR_BUDGET.  Admitted queries
+ * are launched as soon as they are prepared, up to max_inflight at once on
+ * separate streams, so the device does not idle while queries wait
+ * (continuous batching); two queries that need the same prepared execution
+ * run one after the other.  While a server runs, the graphs submitted to it
+ * must stay alive and must not be used by other calls.  No reference
+ * counterpart: the reference runs one app per process
+ * (src/distributed_application.cc:259-356). */
+typedef struct kz_server_s* kz_server;
+typedef struct kz_server_cfg {
+    uint64_t hbm_budget;    /* bytes for prepared executions (0: no limit) */
+    uint32_t max_inflight;  /* concurrent queries on the device (1..16) */
+    uint32_t reserved;
+} kz_server_cfg;
+#define KZ_QUERY_CLIQUE 1   /* k-clique count, k = 3..5 */
+#define KZ_QUERY_CENSUS 2   /* motif census of a symmetric graph */
+#define KZ_PENDING 1        /* kz_server_poll: the query has not finished */
+typedef struct kz_query_result {
+    uint64_t counts[KZ_CENSUS_PATTERNS]; /* clique: counts[0] */
+    double queue_ms;  /* submit -> launch */
+    double run_ms;    /* launch -> result on the host */
+    double total_ms;  /* submit -> result on the host */
+    int status;       /* KZ_OK or the query's error code */
+    int reserved;
+} kz_query_result;
+int kz_server_start(const kz_server_cfg* cfg, kz_server* out);
+int kz_server_submit(kz_server s, kz_graph g, int kind, int k, uint64_t* ticket);
+/* KZ_OK: done, *out filled and the ticket retired; KZ_PENDING: not done
+ * (wait == 0 only); wait != 0 blocks until the query finishes. */
+int kz_server_poll(kz_server s, uint64_t ticket, int wait, kz_query_result* out);
+/* Finishes every submitted query, then stops the worker and frees the
+ * server; prepared executions stay with their graphs. */
+int kz_server_stop(kz_server s);
+
 /* ---- Set primitives, batched -------------------------------------------
  * Many (A, B) list pairs in one launch, one wave64 per pair.  Lists are
  * packed in `values` with offsets list_off[0..num_lists] (strictly

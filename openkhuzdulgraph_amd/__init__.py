@@ -24,7 +24,7 @@ from ._ffi import KzError, KzLibraryMissing, check, lib
 __all__ = [
     "Batch", "CSRGraph", "DeviceGraph", "KzError", "KzLibraryMissing", "init", "shutdown", "device_info",
     "gen_erdos_renyi", "gen_rmat", "gen_chung_lu", "from_edges", "load_dgraph", "set_level_budget",
-    "clique_plan",
+    "clique_plan", "Server",
 ]
 
 _METHODS = {"plain": _ffi.KZ_METHOD_PLAIN, "persistent": _ffi.KZ_METHOD_PERSISTENT}
@@ -216,6 +216,53 @@ class Batch:
     def __del__(self):
         try:
             self.release()
+        except Exception:
+            pass
+
+
+class Server:
+    """Query server on resident graphs (kz_server_*): queries are admitted in
+    arrival order within an HBM budget for prepared executions and launched
+    concurrently on up to max_inflight streams."""
+
+    def __init__(self, hbm_budget: int = 0, max_inflight: int = 4):
+        cfg = _ffi.kz_server_cfg(hbm_budget, max_inflight, 0)
+        self._h = C.c_void_p()
+        check(lib().kz_server_start(C.byref(cfg), C.byref(self._h)))
+        self._graphs = {}  # ticket -> graph (kept alive while the query runs)
+
+    def submit(self, graph: "DeviceGraph", kind: str = "clique", k: int = 4) -> int:
+        code = {"clique": _ffi.KZ_QUERY_CLIQUE, "census": _ffi.KZ_QUERY_CENSUS}[kind]
+        t = C.c_uint64()
+        check(lib().kz_server_submit(self._h, graph._h, code, k, C.byref(t)))
+        self._graphs[int(t.value)] = graph
+        return int(t.value)
+
+    def poll(self, ticket: int, wait: bool = False):
+        """Result dict when the query has finished, else None (wait=False)."""
+        r = _ffi.kz_query_result()
+        rc = lib().kz_server_poll(self._h, ticket, 1 if wait else 0, C.byref(r))
+        if rc == _ffi.KZ_PENDING:
+            return None
+        check(rc)
+        self._graphs.pop(ticket, None)
+        return r.as_dict()
+
+    def stop(self) -> None:
+        if self._h:
+            check(lib().kz_server_stop(self._h))
+            self._h = C.c_void_p()
+            self._graphs.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    def __del__(self):
+        try:
+            self.stop()
         except Exception:
             pass
 

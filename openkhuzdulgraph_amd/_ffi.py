@@ -27,6 +27,9 @@ KZ_MAX_OPS = 4
 KZ_MAX_OUTS = 4
 KZ_MAX_STEP_LISTS = 128
 KZ_CENSUS_PATTERNS = 8
+KZ_QUERY_CLIQUE = 1
+KZ_QUERY_CENSUS = 2
+KZ_PENDING = 1
 CENSUS_PATTERNS = ("wedge", "triangle", "3-path", "3-star", "4-cycle", "tailed-triangle", "diamond", "4-clique")
 
 KZ_METHOD_PLAIN = 0
@@ -109,6 +112,25 @@ class kz_bucket_cfg(C.Structure):
         return {"bound": list(self.bound), "block": list(self.block), "serial": int(self.serial)}
 
 
+class kz_server_cfg(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class kz_query_result(C.Structure):
+    _fields_ = [
+        ("counts", C.c_uint64 * 8),
+        ("queue_ms", C.c_double),
+        ("run_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("status", C.c_int),
+        ("reserved", C.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"counts": [int(x) for x in self.counts], "queue_ms": self.queue_ms, "run_ms": self.run_ms,
+                "total_ms": self.total_ms, "status": int(self.status)}
+
+
 class kz_set_ref(C.Structure):
     _fields_ = [("kind", C.c_uint8), ("a", C.c_uint8), ("b", C.c_uint8), ("reserved", C.c_uint8)]
 
@@ -179,6 +201,10 @@ _SIGS = {
     "kz_set_level_budget": (C.c_int, [C.c_uint64]),
     "kz_clique_workload": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "kz_motif_census": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(kz_run_stats)]),
+    "kz_server_start": (C.c_int, [C.POINTER(kz_server_cfg), C.POINTER(C.c_void_p)]),
+    "kz_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_uint64)]),
+    "kz_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(kz_query_result)]),
+    "kz_server_stop": (C.c_int, [C.c_void_p]),
     "kz_clique_get_buckets": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(kz_bucket_cfg)]),
     "kz_clique_set_buckets": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(kz_bucket_cfg)]),
     "kz_clique_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.POINTER(kz_bucket_cfg), C.POINTER(C.c_double)]),

@@ -70,6 +70,7 @@ struct Ranked {
     uint32_t* touched = nullptr;  // c4_grid * n touched lists
     uint32_t* cursor = nullptr;   // 4 task cursors
     unsigned long long* sums = nullptr;  // 8 accumulators
+    uint64_t bytes = 0;           // device memory held (with the 4-clique job)
     CliqueJob k4;
     ~Ranked() {
         for (void* p : {(void*)rowptr, (void*)col, (void*)deg, (void*)frowptr, (void*)fcol, (void*)tasks, (void*)troot,
@@ -323,6 +324,7 @@ static int build_ranked(DeviceGraph& g, Ranked& R) {
     if (g.nnz >= (uint64_t(1) << 31)) KZ_FAIL(KZ_ERR_UNSUPPORTED, "census relabelling supports < 2^31 list entries");
     R.n = n;
     R.nnz = g.nnz;
+    R.bytes = 4 * (size_t(n) + 1) + 8 * (size_t(n) + 1) + col_alloc_bytes(g.nnz) + 8 * (size_t(n) + 1);
     Scratch tmp;
     uint64_t *k0 = nullptr, *k1 = nullptr;
     uint32_t *rank = nullptr, *order = nullptr, *skip = nullptr, *fdeg = nullptr;
@@ -372,6 +374,7 @@ static int build_ranked(DeviceGraph& g, Ranked& R) {
     KZ_HIP(hipMemcpyAsync(R.h_fdeg.data(), fdeg, 4 * size_t(n), hipMemcpyDeviceToHost, s));
     KZ_HIP(hipMemcpyAsync(&R.fnnz, R.frowptr + n, 8, hipMemcpyDeviceToHost, s));
     KZ_HIP(hipStreamSynchronize(s));
+    R.bytes += col_alloc_bytes(R.fnnz);
     if (hipMalloc(&R.fcol, col_alloc_bytes(R.fnnz)) != hipSuccess) {
         (void)hipGetLastError();
         cleanup();
@@ -417,9 +420,14 @@ static int prepare_passes(Ranked& R) {
     KZ_HIP(hipMalloc(&R.tmid, 4 * std::max<uint64_t>(1, R.fnnz)));
     KZ_HIP(hipMalloc(&R.cursor, 4 * 4));
     KZ_HIP(hipMalloc(&R.sums, 8 * 8));
-    // 4-cycle workgroups: two per CU, each with 8 bytes per vertex of slab
-    R.c4_grid = uint32_t(std::max(1, rt_.num_cus * 2));
+    // 4-cycle workgroups: two per CU, each with 8 bytes per vertex of slab,
+    // fewer when the slabs would take more than a quarter of free memory
+    size_t free_b = 0, total_b = 0;
+    KZ_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per_wg = size_t(std::max<uint32_t>(1, R.n)) * 8;
+    R.c4_grid = uint32_t(std::max<size_t>(1, std::min<size_t>(size_t(rt_.num_cus) * 2, free_b / 4 / per_wg)));
     const size_t slab = size_t(R.c4_grid) * std::max<uint32_t>(1, R.n) * 4;
+    R.bytes += 2 * slab + 8 * (2 * std::max<uint64_t>(1, R.fnnz) + all.size() + 8);
     if (hipMalloc(&R.cnt, slab) != hipSuccess || hipMalloc(&R.touched, slab) != hipSuccess) {
         (void)hipGetLastError();
         KZ_FAIL(KZ_ERR_NOMEM, "4-cycle counter slabs (%zu bytes each)", slab);
@@ -430,6 +438,7 @@ static int prepare_passes(Ranked& R) {
     KZ_HIP(hipStreamSynchronize(rt_.stream));
     std::vector<CliqueQuery> q = {CliqueQuery{R.frowptr, R.fcol, &R.h_fdeg, 4}};
     if (int e = clique_job_prepare(R.k4, q, default_buckets())) return e;
+    R.bytes += R.k4.bytes;
     return KZ_OK;
 }
 
@@ -509,16 +518,11 @@ static int run_census(Ranked& R, uint64_t* sums, double* device_ms) {
 
 }  // namespace
 
-}  // namespace kz
+uint64_t census_bytes(const DeviceGraph& d) {
+    return d.census ? static_cast<const Ranked*>(d.census.get())->bytes : 0;
+}
 
-using namespace kz;
-
-extern "C" {
-
-int kz_motif_census(kz_graph g, uint64_t* counts, kz_run_stats* stats) {
-    if (!g || !counts) KZ_FAIL(KZ_ERR_INVALID, "null argument");
-    if (int e = ensure_ready()) return e;
-    DeviceGraph& d = g->g;
+int motif_census(DeviceGraph& d, uint64_t* counts, kz_run_stats* stats) {
     if (d.oriented) KZ_FAIL(KZ_ERR_INVALID, "the census needs a symmetric graph");
     auto t0 = std::chrono::steady_clock::now();
     Ranked* R = nullptr;
@@ -560,4 +564,10 @@ int kz_motif_census(kz_graph g, uint64_t* counts, kz_run_stats* stats) {
     return KZ_OK;
 }
 
-}  // extern "C"
+}  // namespace kz
+
+extern "C" int kz_motif_census(kz_graph g, uint64_t* counts, kz_run_stats* stats) {
+    if (!g || !counts) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    if (int e = kz::ensure_ready()) return e;
+    return kz::motif_census(g->g, counts, stats);
+}

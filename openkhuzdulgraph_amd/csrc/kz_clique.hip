@@ -658,6 +658,7 @@ static int prepare_hubs(CliqueJob& x, int k, const std::vector<uint32_t>& hv, co
     KZ_HIP(hipMalloc(&dm, sizeof(HubMeta) * std::max<uint32_t>(1, hp.nhubs)));
     KZ_HIP(hipMalloc(&di, 4 * std::max<uint32_t>(1, hp.nhubs)));
     KZ_HIP(hipMalloc(&dr, 4 * std::max<uint32_t>(1, hp.nhubs)));
+    x.bytes += (sizeof(HubMeta) + 8) * std::max<uint32_t>(1, hp.nhubs);
     x.hub_meta[k] = dm;
     x.hub_items[k] = di;
     x.hub_rows[k] = dr;
@@ -667,6 +668,7 @@ static int prepare_hubs(CliqueJob& x, int k, const std::vector<uint32_t>& hv, co
     if (k >= 4) {
         unsigned long long* s = nullptr;
         KZ_HIP(hipMalloc(&s, std::max<uint64_t>(1, slab) * 8));
+        x.bytes += std::max<uint64_t>(1, slab) * 8;
         x.slabs[k] = s;
     }
     x.hubplan[k] = hp;
@@ -740,6 +742,8 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
     }
     KZ_HIP(hipMalloc(&x.d_cursor, sizeof(uint32_t) * std::max(1, cursor)));
     KZ_HIP(hipMalloc(&x.d_result, sizeof(unsigned long long) * std::max<size_t>(1, queries.size())));
+    x.bytes += sizeof(QueryRef) * qr.size() + 4 * std::max<size_t>(1, hv.size()) * (queries.size() > 1 ? 2 : 1) +
+               4 * size_t(std::max(1, cursor)) + 8 * std::max<size_t>(1, queries.size());
     // capture
     Runtime& R = rt();
     hipStream_t main = R.stream;
@@ -790,6 +794,13 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
     x.graph = graph;
     KZ_HIP(hipGraphInstantiate(&x.exec, graph, nullptr, nullptr, 0));
     x.ready = true;
+    return KZ_OK;
+}
+
+int clique_job_launch(CliqueJob& x, hipStream_t s, unsigned long long* host_results) {
+    KZ_HIP(hipGraphLaunch(x.exec, s));
+    KZ_HIP(hipMemcpyAsync(host_results, x.d_result, sizeof(unsigned long long) * std::max<uint32_t>(1, x.nqueries),
+                          hipMemcpyDeviceToHost, s));
     return KZ_OK;
 }
 

@@ -115,7 +115,7 @@ static std::shared_ptr<void> wrap_job(CliqueJob* j) {
 }
 
 // Prepared single-graph job for (graph, k), built on first use.
-static int graph_job(DeviceGraph& d, int k, CliqueJob** out) {
+int graph_job(DeviceGraph& d, int k, CliqueJob** out) {
     if (int e = ensure_forward(d)) return e;
     auto& slot = d.clique_exec[k];
     if (!slot) {

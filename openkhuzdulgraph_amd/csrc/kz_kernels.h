@@ -203,6 +203,7 @@ struct CliqueJob {
     std::map<int, uint32_t*> hub_items;    // exclusive prefix of row chunks per hub
     std::map<int, uint32_t*> hub_rows;     // exclusive prefix of rows per hub
     std::map<int, unsigned long long*> slabs;  // per k, HBM bitmap rows
+    uint64_t bytes = 0;                        // device memory this job holds
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~CliqueJob();
@@ -219,6 +220,17 @@ __global__ void fwd_copy_kernel(const uint64_t* __restrict__ rowptr, const uint3
 
 int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, const BucketCfg& cfg);
 int clique_job_run(CliqueJob& x, uint64_t* counts, double* device_ms);
+// Launches a prepared job on stream s and copies its per-query counts to
+// host_results (pinned, nqueries entries) on the same stream; no wait.
+int clique_job_launch(CliqueJob& x, hipStream_t s, unsigned long long* host_results);
+// Prepared single-graph job for (graph, k), built on first use and kept in
+// the graph (kz_count.hip).
+int graph_job(DeviceGraph& d, int k, CliqueJob** out);
+// Motif census of a symmetric resident graph; the relabelled copy and
+// counter slabs are kept in the graph (kz_census.hip).  census_bytes is the
+// device memory that state holds (0 before the first call).
+int motif_census(DeviceGraph& d, uint64_t* counts, kz_run_stats* stats);
+uint64_t census_bytes(const DeviceGraph& d);
 int clique_workload(const CliqueView& gv, uint64_t* out6);
 int ensure_forward(DeviceGraph& g);
 
