@@ -21,6 +21,8 @@ Other workloads (each prints its own JSON line; the default is the one above):
     --workload batch    many seeded ER/RMAT graphs (seed 1000+i, n ~ U[2^10,
                         2^18], average degree ~ U[4, 64]), triangle, 4- and
                         5-clique of every graph in one hipGraph launch
+    --workload gen      RMAT generation + clean-up + orientation on the host
+                        and on the GPU (kz_gpu_gen_rmat), same digest
     --workload serve    a seeded Poisson stream of queries through kz_server:
                         triangle and 4-clique on a large RMAT graph, 5-clique
                         and motif census on a pool of batch-workload graphs,
@@ -82,10 +84,11 @@ def cpu_baseline(g, budget_s: float, threads: int) -> dict:
     }
 
 
-def pool_graph(kz, i: int):
+def pool_graph(kz, i: int, gpu: bool = False, orient: bool = False):
     """Graph i of the batch workload (seed 1000 + i): even i Erdos-Renyi,
     odd i RMAT (.57/.19/.19/.05); n ~ U[2^10, 2^18], average degree ~ U[4, 64]
-    (RMAT: scale round(log2 n), edge factor round(degree / 2))."""
+    (RMAT: scale round(log2 n), edge factor round(degree / 2)).  Host CSR, or
+    with gpu=True the same graph built in HBM (a DeviceGraph)."""
     import math
 
     import numpy as np
@@ -93,9 +96,14 @@ def pool_graph(kz, i: int):
     rng = np.random.default_rng(1000 + i)
     n = int(rng.integers(2**10, 2**18 + 1))
     deg = float(rng.uniform(4, 64))
-    if i % 2 == 0:
-        return kz.gen_erdos_renyi(n, int(n * deg / 2), 1000 + i)
-    return kz.gen_rmat(max(10, round(math.log2(n))), max(2, round(deg / 2)), 0.57, 0.19, 0.19, 1000 + i)
+    if gpu:
+        if i % 2 == 0:
+            return kz.gpu_gen_erdos_renyi(n, int(n * deg / 2), 1000 + i, orient)
+        return kz.gpu_gen_rmat(max(10, round(math.log2(n))), max(2, round(deg / 2)), 0.57, 0.19, 0.19, 1000 + i,
+                               orient)
+    g = (kz.gen_erdos_renyi(n, int(n * deg / 2), 1000 + i) if i % 2 == 0 else
+         kz.gen_rmat(max(10, round(math.log2(n))), max(2, round(deg / 2)), 0.57, 0.19, 0.19, 1000 + i))
+    return g.oriented() if orient else g
 
 
 def base_line(metric: str, value, unit: str, steps: int, warmup: int, ms: float, higher: bool, config: dict) -> dict:
@@ -145,11 +153,14 @@ def bench_census(kz, args) -> int:
 
 def bench_batch(kz, args) -> int:
     t0 = time.perf_counter()
-    graphs = [pool_graph(kz, i).oriented() for i in range(args.graphs)]
-    ent = sum(g.num_entries for g in graphs)
-    log(f"generated {args.graphs} graphs ({ent} oriented entries) in {time.perf_counter() - t0:.1f} s")
     kz.init(0)
-    dgs = [g.to_device() for g in graphs]
+    if args.host_gen:
+        dgs = [pool_graph(kz, i, orient=True).to_device() for i in range(args.graphs)]
+    else:
+        dgs = [pool_graph(kz, i, gpu=True, orient=True) for i in range(args.graphs)]
+    ent = sum(dg.num_entries for dg in dgs)
+    log(f"generated {args.graphs} graphs ({ent} oriented entries) on the {'host' if args.host_gen else 'GPU'} "
+        f"in {time.perf_counter() - t0:.1f} s")
     ks = [3, 4, 5]
     b = kz.Batch(dgs, ks)
     t0 = time.perf_counter()
@@ -173,11 +184,12 @@ def bench_batch(kz, args) -> int:
             for j, k in enumerate(ks):
                 if dgs[i].count_cliques(k, "persistent") != int(first[i, j]):
                     raise SystemExit(f"batch differs from the single-graph run: graph {i} k {k}")
-        small = sorted(range(args.graphs), key=lambda i: graphs[i].num_entries)[:4]
+        small = sorted(range(args.graphs), key=lambda i: dgs[i].num_entries)[:4]
         apps = {3: "kautomine_triangle", 4: "kautomine_four_clique", 5: "kautomine_five_clique"}
         for i in small:
+            h = dgs[i].download()
             for j, k in enumerate(ks):
-                if oracle.run_app(apps[k], graphs[i].rowptr, graphs[i].col)[1][0] != int(first[i, j]):
+                if oracle.run_app(apps[k], h.rowptr, h.col)[1][0] != int(first[i, j]):
                     raise SystemExit(f"batch differs from the oracle: graph {i} k {k}")
         check = {"single_runs": len(idx), "oracle_graphs": len(small)}
         log(f"checks: {check}")
@@ -196,14 +208,17 @@ def bench_serve(kz, args) -> int:
     import numpy as np
 
     t0 = time.perf_counter()
-    big = kz.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed).oriented()
-    pool_sym = [pool_graph(kz, i) for i in range(args.graphs)]
-    pool_ori = [g.oriented() for g in pool_sym]
-    log(f"generated {big} and {args.graphs} pool graphs in {time.perf_counter() - t0:.1f} s")
     kz.init(0)
-    dbig = big.to_device()
-    dsym = [g.to_device() for g in pool_sym]
-    dori = [g.to_device() for g in pool_ori]
+    if args.host_gen:
+        dbig = kz.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed).oriented().to_device()
+        dsym = [pool_graph(kz, i).to_device() for i in range(args.graphs)]
+        dori = [pool_graph(kz, i, orient=True).to_device() for i in range(args.graphs)]
+    else:
+        dbig = kz.gpu_gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed, orient=True)
+        dsym = [pool_graph(kz, i, gpu=True) for i in range(args.graphs)]
+        dori = [pool_graph(kz, i, gpu=True, orient=True) for i in range(args.graphs)]
+    log(f"generated the large graph ({dbig.num_vertices} vertices, {dbig.num_entries} oriented entries) and "
+        f"{args.graphs} pool graphs in {time.perf_counter() - t0:.1f} s")
     # query kinds: (graph, kind, k); large graph: triangle, 4-clique; pool:
     # 5-clique (oriented copy) and the motif census (symmetric copy)
     def kinds(i):
@@ -266,10 +281,44 @@ def bench_serve(kz, args) -> int:
     line = base_line("serving exact counts: queries/s and p50/p99 latency, 1 MI355X", best["queries_per_s"],
                      "queries/s", len(rates), 0, best["p50_ms"], True,
                      {"workload": f"serve_rmat{args.scale}_ef{args.edge_factor}_pool{args.graphs}",
-                      "large_graph_vertices": big.num_vertices, "large_graph_oriented_edges": big.num_entries,
+                      "large_graph_vertices": dbig.num_vertices, "large_graph_oriented_edges": dbig.num_entries,
                       "pool_graphs": args.graphs, "budget_bytes": budget, "max_inflight": 8,
                       "mix": "large: triangle, 4-clique; pool: 5-clique, motif census (uniform)"})
     line["per_rate"] = per_rate
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_gen(kz, args) -> int:
+    """RMAT(scale, edge factor) generation + clean-up + degree orientation on
+    the host (OpenMP) and on the GPU; both must give the same CSR."""
+    kz.init(0)
+    ms = []
+    for _ in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        dg = kz.gpu_gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed, orient=True)
+        ms.append((time.perf_counter() - t0) * 1e3)
+        n, m = dg.num_vertices, dg.num_entries
+        last = dg
+    gpu_ms = sum(ms[args.warmup:]) / args.steps
+    log(f"GPU: {n} vertices, {m} oriented entries, {gpu_ms:.1f} ms")
+    host = None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        h = kz.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed).oriented()
+        host_s = time.perf_counter() - t0
+        same = h.digest() == last.download().digest()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        host = {"seconds": host_s, "threads": threads, "same_digest": same}
+        log(f"host: {host}")
+        if not same:
+            raise SystemExit("GPU and host graphs differ")
+    draws = args.edge_factor << args.scale
+    line = base_line("graph generation + preprocessing: draws/s, 1 MI355X", draws / (gpu_ms / 1e3), "draws/s",
+                     args.steps, args.warmup, gpu_ms, True,
+                     {"workload": f"rmat{args.scale}_ef{args.edge_factor}_generate_clean_orient", "draws": draws,
+                      "vertices": n, "oriented_edges": m})
+    line["host"] = host
     print(json.dumps(line), flush=True)
     return 0
 
@@ -289,7 +338,10 @@ def main() -> int:
     ap.add_argument("--no-dgraph", action="store_true", help="skip the .dgraph round trip")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
-    ap.add_argument("--workload", choices=["clique", "census", "batch", "serve"], default="clique")
+    ap.add_argument("--workload", choices=["clique", "census", "batch", "serve", "gen"], default="clique")
+    ap.add_argument("--host-gen", action="store_true", help="batch/serve: generate on the host, not the GPU")
+    ap.add_argument("--gpu-gen", action="store_true",
+                    help="clique: build the graph on the GPU (no .dgraph round trip; large scales)")
     ap.add_argument("--graphs", type=int, default=256, help="batch/serve: number of pool graphs")
     ap.add_argument("--check", action="store_true", help="census/batch: also check against the oracle")
     ap.add_argument("--rates", default="4,8,16,32", help="serve: arrival rates (queries/s)")
@@ -319,14 +371,24 @@ def main() -> int:
         return bench_batch(kz, args)
     if args.workload == "serve":
         return bench_serve(kz, args)
+    if args.workload == "gen":
+        return bench_gen(kz, args)
 
     t0 = time.perf_counter()
-    sym = kz.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
-    g = sym.oriented()
-    del sym
-    log(f"generated {g} in {time.perf_counter() - t0:.1f} s")
-    digest = g.digest()
-    if not args.no_dgraph:
+    if args.gpu_gen:
+        kz.init(0)
+        dg = kz.gpu_gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed, orient=True)
+        log(f"built on the GPU: {dg.num_vertices} vertices, {dg.num_entries} oriented entries in "
+            f"{time.perf_counter() - t0:.1f} s")
+        g = dg.download() if not args.no_cpu else None
+        digest = g.digest() if g is not None else 0
+    else:
+        sym = kz.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
+        g = sym.oriented()
+        del sym
+        log(f"generated {g} in {time.perf_counter() - t0:.1f} s")
+        digest = g.digest()
+    if not args.no_dgraph and not args.gpu_gen:
         with tempfile.TemporaryDirectory() as td:
             path = os.path.join(td, f"rmat{args.scale}.dgraph")
             g.store(path)
@@ -338,7 +400,8 @@ def main() -> int:
 
     kz.init(0)
     dev = kz.device_info()
-    dg = g.to_device()
+    if not args.gpu_gen:
+        dg = g.to_device()
     work = dg.clique_workload()
     log(f"workload: {work}")
 
@@ -413,13 +476,13 @@ def main() -> int:
             "workload": f"rmat{args.scale}_ef{args.edge_factor}_k{args.k}_clique_count",
             "generator": "rmat a/b/c/d=0.57/0.19/0.19/0.05, symmetrised, deduped, no loops/isolated, degree-oriented",
             "scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed, "k": args.k,
-            "vertices": g.num_vertices, "oriented_edges": g.num_entries, "max_out_degree": g.max_degree,
+            "vertices": dg.num_vertices, "oriented_edges": dg.num_entries, "max_out_degree": dg.max_degree,
             "csr_digest": f"{digest:016x}",
         },
         "count": count,
         "seconds": t_pers,
         "counts_per_s": count / t_pers,
-        "edges_per_s": g.num_entries / t_pers,
+        "edges_per_s": dg.num_entries / t_pers,
         "merge_model_bytes": B4,
         "plain": plain,
         "speedup_vs_plain": (plain["seconds"] / t_pers) if plain else None,

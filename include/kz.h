@@ -93,14 +93,15 @@ int kz_gen_chung_lu(uint32_t n, uint64_t num_draws, double exponent, uint32_t ma
  * 27-124).  `sym` must be symmetric. */
 int kz_orient_by_degree(const kz_csr* sym, kz_csr* out);
 
+
 /* Structural checks of toolkits/check_graph.cc:44-68: strictly increasing
  * lists, no self-loops, degree sum = entries; symmetric graphs also need
  * every reverse edge, oriented graphs every dst > src.  Returns KZ_OK or
  * KZ_ERR_INVALID with a message. */
 int kz_csr_check(const kz_csr* g);
 
-/* FNV-1a 64-bit digest of the CSR arrays (used to prove that two hosts
- * rebuilt the same graph). */
+/* 64-bit digest (splitmix64 fold of n, entries, rowptr[], col[]; thread-count
+ * independent) used to prove that two builds produced the same graph. */
 uint64_t kz_csr_digest(const kz_csr* g);
 
 /* ------------------------------------------------------------------------ */
@@ -136,6 +137,27 @@ typedef struct kz_graph_s* kz_graph;
 int kz_graph_upload(const kz_csr* g, kz_graph* out);
 int kz_graph_release(kz_graph g);
 int kz_graph_num_vertices(kz_graph g, uint32_t* out);
+/* Vertices, stored entries, largest out-degree and whether it is oriented. */
+int kz_graph_info(kz_graph g, uint32_t* num_vertices, uint64_t* num_entries, uint32_t* max_degree, int* oriented);
+
+/* The same generators on the GPU, building the graph directly in HBM:
+ * identical draws and the same CSR bit for bit as kz_gen_* (+
+ * kz_orient_by_degree when orient != 0), with the clean-up (self-loops,
+ * duplicates, isolated vertices, ascending relabel;
+ * datasets/live_journal/standardize.cc:66-114) and the orientation
+ * (toolkits/to_oriented_dataset_converter.cc:27-124) done by radix sorts.
+ * Needs about 24 bytes of device memory per draw while it runs. */
+int kz_gpu_gen_erdos_renyi(uint32_t n, uint64_t m, uint64_t seed, int orient, kz_graph* out);
+int kz_gpu_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed, int orient,
+                    kz_graph* out);
+int kz_gpu_gen_chung_lu(uint32_t n, uint64_t num_draws, double exponent, uint32_t max_weight, uint64_t seed,
+                        int orient, kz_graph* out);
+/* Copies a resident graph back into a host CSR (released with
+ * kz_csr_release); e.g. to write it as .dgraph. */
+int kz_graph_download(kz_graph g, kz_csr* out);
+/* Largest number of keys one radix-sort call of the GPU generators takes
+ * (default 2^30; smaller values exercise the chunked path in tests). */
+int kz_set_sort_chunk(uint64_t items);
 
 /* ------------------------------------------------------------------------ */
 /* Counting                                                                  */

@@ -24,7 +24,7 @@ from ._ffi import KzError, KzLibraryMissing, check, lib
 __all__ = [
     "Batch", "CSRGraph", "DeviceGraph", "KzError", "KzLibraryMissing", "init", "shutdown", "device_info",
     "gen_erdos_renyi", "gen_rmat", "gen_chung_lu", "from_edges", "load_dgraph", "set_level_budget",
-    "clique_plan", "Server",
+    "clique_plan", "Server", "gpu_gen_erdos_renyi", "gpu_gen_rmat", "gpu_gen_chung_lu",
 ]
 
 _METHODS = {"plain": _ffi.KZ_METHOD_PLAIN, "persistent": _ffi.KZ_METHOD_PERSISTENT}
@@ -128,6 +128,33 @@ def from_edges(src, dst, num_vertices_hint: int) -> CSRGraph:
 
 def load_dgraph(path: str) -> CSRGraph:
     return _new(lib().kz_dgraph_load, str(path).encode())
+
+
+def _gpu_new(fn, *args) -> "DeviceGraph":
+    h = C.c_void_p()
+    check(fn(*args, C.byref(h)))
+    return DeviceGraph(h)
+
+
+def gpu_gen_erdos_renyi(n: int, m: int, seed: int, orient: bool = False) -> "DeviceGraph":
+    """gen_erdos_renyi (and .oriented()) built on the GPU, resident in HBM."""
+    return _gpu_new(lib().kz_gpu_gen_erdos_renyi, n, m, seed, int(orient))
+
+
+def gpu_gen_rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+                 seed: int = 1, orient: bool = False) -> "DeviceGraph":
+    """gen_rmat (and .oriented()) built on the GPU, resident in HBM."""
+    return _gpu_new(lib().kz_gpu_gen_rmat, scale, edge_factor, a, b, c, seed, int(orient))
+
+
+def gpu_gen_chung_lu(n: int, num_draws: int, exponent: float, max_weight: int, seed: int,
+                     orient: bool = False) -> "DeviceGraph":
+    """gen_chung_lu (and .oriented()) built on the GPU, resident in HBM."""
+    return _gpu_new(lib().kz_gpu_gen_chung_lu, n, num_draws, exponent, max_weight, seed, int(orient))
+
+
+def set_sort_chunk(items: int) -> None:
+    check(lib().kz_set_sort_chunk(int(items)))
 
 
 def init(device: int = 0) -> None:
@@ -291,6 +318,33 @@ class DeviceGraph:
             self.release()
         except Exception:
             pass
+
+    def _info(self):
+        n, m, d, o = C.c_uint32(), C.c_uint64(), C.c_uint32(), C.c_int()
+        check(lib().kz_graph_info(self._h, C.byref(n), C.byref(m), C.byref(d), C.byref(o)))
+        return int(n.value), int(m.value), int(d.value), bool(o.value)
+
+    @property
+    def num_vertices(self) -> int:
+        return self._info()[0]
+
+    @property
+    def num_entries(self) -> int:
+        return self._info()[1]
+
+    @property
+    def max_degree(self) -> int:
+        return self._info()[2]
+
+    @property
+    def oriented_flag(self) -> bool:
+        return self._info()[3]
+
+    def download(self) -> CSRGraph:
+        """Copy of the resident graph as a host CSR."""
+        out = _ffi.kz_csr()
+        check(lib().kz_graph_download(self._h, C.byref(out)))
+        return CSRGraph(out)
 
     def count_cliques(self, k: int, method: str = "persistent") -> int:
         out = C.c_uint64()

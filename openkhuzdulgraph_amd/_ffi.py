@@ -193,6 +193,15 @@ _SIGS = {
     "kz_graph_upload": (C.c_int, [C.POINTER(kz_csr), C.POINTER(C.c_void_p)]),
     "kz_graph_release": (C.c_int, [C.c_void_p]),
     "kz_graph_num_vertices": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "kz_gpu_gen_erdos_renyi": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
+    "kz_gpu_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
+                                  C.POINTER(C.c_void_p)]),
+    "kz_gpu_gen_chung_lu": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_uint32, C.c_uint64, C.c_int,
+                                      C.POINTER(C.c_void_p)]),
+    "kz_graph_download": (C.c_int, [C.c_void_p, C.POINTER(kz_csr)]),
+    "kz_graph_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_int)]),
+    "kz_set_sort_chunk": (C.c_int, [C.c_uint64]),
     "kz_count_cliques": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_uint64), C.POINTER(kz_run_stats)]),
     "kz_plan_clique": (C.c_int, [C.c_int, C.POINTER(kz_plan_desc)]),
     "kz_plan_validate": (C.c_int, [C.POINTER(kz_plan_desc)]),

@@ -246,8 +246,29 @@ int kz_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double
 
 int kz_gen_chung_lu(uint32_t n, uint64_t num_draws, double exponent, uint32_t max_weight,
                     uint64_t seed, kz_csr* out) {
-    if (!out || n < 2 || exponent <= 1.0 || max_weight < 1)
-        KZ_FAIL(KZ_ERR_INVALID, "chung_lu parameters out of range");
+    if (!out) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    std::vector<double> cdf;
+    if (int e = chung_lu_cdf(n, num_draws, exponent, max_weight, &cdf)) return e;
+    const double total = cdf[n];
+    const uint64_t key = stream_key(seed, 0x434C55ULL);
+    auto pick = [&](uint32_t r32) {
+        double x = (double(r32) + 0.5) * (total / 4294967296.0);
+        uint64_t k = uint64_t(std::upper_bound(cdf.begin() + 1, cdf.end(), x) - (cdf.begin() + 1));
+        return uint32_t(k < n ? k : n - 1);
+    };
+    auto fn = [&](uint64_t i, uint32_t* u, uint32_t* v) {
+        uint64_t r = draw64(key, i);
+        *u = pick(uint32_t(r));
+        *v = pick(uint32_t(r >> 32));
+    };
+    return clean_symmetric(num_draws, n, fn, out);
+}
+
+}  // extern "C"
+
+int kz::chung_lu_cdf(uint32_t n, uint64_t num_draws, double exponent, uint32_t max_weight,
+                     std::vector<double>* cdf_out) {
+    if (n < 2 || exponent <= 1.0 || max_weight < 1) KZ_FAIL(KZ_ERR_INVALID, "chung_lu parameters out of range");
     // Weights w_i = max_weight * (i0 / (i + i0))^(1/(exponent-1)); i0 is
     // chosen so that sum(w) = 2 * num_draws (expected degree sum).
     const double alpha = 1.0 / (exponent - 1.0);
@@ -265,24 +286,14 @@ int kz_gen_chung_lu(uint32_t n, uint64_t num_draws, double exponent, uint32_t ma
         (weight_sum(mid) < target ? lo : hi) = mid;
     }
     const double i0 = hi;
-    std::vector<double> cdf(size_t(n) + 1);
-    cdf[0] = 0;
+    std::vector<double>& cdf = *cdf_out;
+    cdf.assign(size_t(n) + 1, 0.0);
     for (uint64_t i = 0; i < n; ++i)
         cdf[i + 1] = cdf[i] + double(max_weight) * std::pow(i0 / (double(i) + i0), alpha);
-    const double total = cdf[n];
-    const uint64_t key = stream_key(seed, 0x434C55ULL);
-    auto pick = [&](uint32_t r32) {
-        double x = (double(r32) + 0.5) * (total / 4294967296.0);
-        uint64_t k = uint64_t(std::upper_bound(cdf.begin() + 1, cdf.end(), x) - (cdf.begin() + 1));
-        return uint32_t(k < n ? k : n - 1);
-    };
-    auto fn = [&](uint64_t i, uint32_t* u, uint32_t* v) {
-        uint64_t r = draw64(key, i);
-        *u = pick(uint32_t(r));
-        *v = pick(uint32_t(r >> 32));
-    };
-    return clean_symmetric(num_draws, n, fn, out);
+    return KZ_OK;
 }
+
+extern "C" {
 
 int kz_orient_by_degree(const kz_csr* sym, kz_csr* out) {
     if (!sym || !out || !sym->rowptr) KZ_FAIL(KZ_ERR_INVALID, "null graph");

@@ -164,6 +164,15 @@ int kz_graph_release(kz_graph g) {
     return KZ_OK;
 }
 
+int kz_graph_info(kz_graph g, uint32_t* num_vertices, uint64_t* num_entries, uint32_t* max_degree, int* oriented) {
+    if (!g) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    if (num_vertices) *num_vertices = g->g.n;
+    if (num_entries) *num_entries = g->g.nnz;
+    if (max_degree) *max_degree = g->g.max_degree;
+    if (oriented) *oriented = g->g.oriented;
+    return KZ_OK;
+}
+
 int kz_graph_num_vertices(kz_graph g, uint32_t* out) {
     if (!g || !out) KZ_FAIL(KZ_ERR_INVALID, "null argument");
     *out = g->g.n;
