@@ -247,11 +247,16 @@ def bench_serve(kz, args) -> int:
             t_start = time.perf_counter()
             next_t = t_start
             outstanding, lat, done, mismatches = {}, [], 0, 0
+            next_log = t_start + 30.0
             while True:
                 now = time.perf_counter()
+                if now >= next_log:
+                    log(f"rate {rate}/s: {done} done, {len(outstanding)} outstanding after {now - t_start:.0f} s")
+                    next_log += 30.0
                 if now >= next_t and now - t_start < args.seconds:
                     i = int(rng.integers(0, args.graphs))
-                    g, kind, k, name = kinds(i)[int(rng.integers(0, 4))]
+                    big_q = rng.random() < args.large_share
+                    g, kind, k, name = kinds(i)[int(rng.integers(0, 2)) + (0 if big_q else 2)]
                     outstanding[srv.submit(g, kind, k or 4)] = (name, kind, k)
                     next_t += float(rng.exponential(1.0 / rate))
                     continue
@@ -262,6 +267,9 @@ def bench_serve(kz, args) -> int:
                     key = outstanding.pop(t)
                     if r["status"] != 0 or r["counts"] != alone[key]:
                         mismatches += 1
+                        if mismatches <= 10:
+                            log(f"mismatch {key}: status {r['status']} counts {r['counts'][:3]} "
+                                f"alone {alone[key][:3]} total_ms {r['total_ms']:.1f}")
                     lat.append(r["total_ms"])
                     done += 1
                 if now - t_start >= args.seconds and not outstanding:
@@ -283,7 +291,8 @@ def bench_serve(kz, args) -> int:
                      {"workload": f"serve_rmat{args.scale}_ef{args.edge_factor}_pool{args.graphs}",
                       "large_graph_vertices": dbig.num_vertices, "large_graph_oriented_edges": dbig.num_entries,
                       "pool_graphs": args.graphs, "budget_bytes": budget, "max_inflight": 8,
-                      "mix": "large: triangle, 4-clique; pool: 5-clique, motif census (uniform)"})
+                      "mix": f"{args.large_share:.2f} large-graph queries (triangle, 4-clique), the rest pool "
+                             "queries (5-clique, motif census), each pair uniform"})
     line["per_rate"] = per_rate
     print(json.dumps(line), flush=True)
     return 0
@@ -347,6 +356,7 @@ def main() -> int:
     ap.add_argument("--rates", default="4,8,16,32", help="serve: arrival rates (queries/s)")
     ap.add_argument("--seconds", type=float, default=8.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="serve: HBM budget for prepared executions")
+    ap.add_argument("--large-share", type=float, default=0.5, help="serve: share of queries on the large graph")
     args = ap.parse_args()
     if args.copy_probe:
         import torch

@@ -254,8 +254,10 @@ int kz_motif_census(kz_graph g, uint64_t* counts, kz_run_stats* stats);
  * are launched as soon as they are prepared, up to max_inflight at once on
  * separate streams, so the device does not idle while queries wait
  * (continuous batching); two queries that need the same prepared execution
- * run one after the other.  While a server runs, the graphs submitted to it
- * must stay alive and must not be used by other calls.  No reference
+ * run one after the other.  A prepared execution already kept by a graph
+ * (from a direct call) is adopted and accounted when a query needs it.  Until
+ * kz_server_stop returns, the graphs submitted to a server must stay alive
+ * and must not be used by other calls.  No reference
  * counterpart: the reference runs one app per process
  * (src/distributed_application.cc:259-356). */
 typedef struct kz_server_s* kz_server;
@@ -280,8 +282,8 @@ int kz_server_submit(kz_server s, kz_graph g, int kind, int k, uint64_t* ticket)
 /* KZ_OK: done, *out filled and the ticket retired; KZ_PENDING: not done
  * (wait == 0 only); wait != 0 blocks until the query finishes. */
 int kz_server_poll(kz_server s, uint64_t ticket, int wait, kz_query_result* out);
-/* Finishes every submitted query, then stops the worker and frees the
- * server; prepared executions stay with their graphs. */
+/* Finishes every submitted query, stops the worker, releases the prepared
+ * executions the server holds and frees the server. */
 int kz_server_stop(kz_server s);
 
 /* ---- Set primitives, batched -------------------------------------------

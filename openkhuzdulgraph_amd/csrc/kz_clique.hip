@@ -683,48 +683,76 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
     if (int e = set_lds_attrs()) return e;
     x.cfg = cfg;
     x.nqueries = uint32_t(queries.size());
-    struct T {
-        uint32_t d, v, q;
-    };
-    std::map<std::pair<int, int>, std::vector<T>> lists;
+    // Tasks grouped per (k, bucket), each group ordered by degree descending
+    // and, within a degree, by query and vertex: a counting sort on the
+    // degree (<= kMaxBitmapDegree), two passes over the roots.
+    constexpr int kLists = 3 * kNumBuckets;
+    const uint32_t D = kMaxBitmapDegree;
+    std::vector<std::vector<uint64_t>> cnt(kLists);
+    auto list_of = [&](int k, uint32_t d) { return (k - 3) * kNumBuckets + int(bucket_of(cfg, d)); };
     for (uint32_t qi = 0; qi < queries.size(); ++qi) {
         const CliqueQuery& Q = queries[qi];
         if (Q.k < 3 || Q.k > 5) KZ_FAIL(KZ_ERR_UNSUPPORTED, "clique size %d outside 3..5", Q.k);
         const std::vector<uint32_t>& fd = *Q.fdeg;
         for (uint32_t v = 0; v < fd.size(); ++v) {
-            if (fd[v] < uint32_t(Q.k - 1)) continue;
-            if (fd[v] > kMaxBitmapDegree)
+            const uint32_t d = fd[v];
+            if (d < uint32_t(Q.k - 1)) continue;
+            if (d > D)
                 KZ_FAIL(KZ_ERR_UNSUPPORTED,
-                        "persistent clique form needs out-degree <= %u (found %u); orient the graph by degree",
-                        kMaxBitmapDegree, fd[v]);
-            lists[{Q.k, int(bucket_of(cfg, fd[v]))}].push_back(T{fd[v], v, qi});
+                        "persistent clique form needs out-degree <= %u (found %u); orient the graph by degree", D, d);
+            auto& c = cnt[list_of(Q.k, d)];
+            if (c.empty()) c.assign(size_t(D) + 1, 0);
+            c[d]++;
         }
     }
-    std::vector<uint32_t> hv, hq;
+    // list starts, then per degree (descending) the next free position
+    std::vector<uint64_t> lstart(kLists + 1, 0);
+    for (int l = 0; l < kLists; ++l) {
+        uint64_t tot = 0;
+        for (uint64_t c : cnt[l]) tot += c;
+        lstart[l + 1] = lstart[l] + tot;
+    }
+    std::vector<std::vector<uint64_t>> pos(kLists);
+    for (int l = 0; l < kLists; ++l) {
+        if (cnt[l].empty()) continue;
+        pos[l].assign(size_t(D) + 1, 0);
+        uint64_t run = lstart[l];
+        for (uint32_t d = D + 1; d-- > 0;) {
+            pos[l][d] = run;
+            run += cnt[l][d];
+        }
+    }
+    const uint64_t ntask = lstart[kLists];
+    std::vector<uint32_t> hv(ntask), hq(ntask);
+    for (uint32_t qi = 0; qi < queries.size(); ++qi) {
+        const CliqueQuery& Q = queries[qi];
+        const std::vector<uint32_t>& fd = *Q.fdeg;
+        for (uint32_t v = 0; v < fd.size(); ++v) {
+            const uint32_t d = fd[v];
+            if (d < uint32_t(Q.k - 1)) continue;
+            const uint64_t at = pos[list_of(Q.k, d)][d]++;
+            hv[at] = v;
+            hq[at] = qi;
+        }
+    }
     x.groups.clear();
     int cursor = 0;
-    for (auto& kv : lists) {
-        auto& L = kv.second;
-        std::stable_sort(L.begin(), L.end(), [](const T& a, const T& b) { return a.d > b.d; });
+    for (int l = 0; l < kLists; ++l) {
+        if (lstart[l + 1] == lstart[l]) continue;
         CliqueJob::Group gr;
-        gr.k = kv.first.first;
-        gr.bucket = kv.first.second;
-        gr.first = hv.size();
-        gr.count = L.size();
-        gr.dmax = gr.bucket == 0 ? 64 : (gr.bucket == kNumBuckets - 1 ? L.front().d : cfg.hi[gr.bucket]);
+        gr.k = 3 + l / kNumBuckets;
+        gr.bucket = l % kNumBuckets;
+        gr.first = lstart[l];
+        gr.count = lstart[l + 1] - lstart[l];
+        const uint32_t dfirst = (*queries[hq[gr.first]].fdeg)[hv[gr.first]];  // the largest in the group
+        gr.dmax = gr.bucket == 0 ? 64 : (gr.bucket == kNumBuckets - 1 ? dfirst : cfg.hi[gr.bucket]);
         gr.cursor = cursor;
         cursor += (gr.bucket == kNumBuckets - 1) ? 2 : 1;  // hubs: rows + count cursors
-        for (const T& t : L) {
-            hv.push_back(t.v);
-            hq.push_back(t.q);
-        }
         if (gr.bucket == kNumBuckets - 1) {
-            std::vector<uint32_t> v, q, d;
-            for (const T& t : L) {
-                v.push_back(t.v);
-                q.push_back(t.q);
-                d.push_back(t.d);
-            }
+            std::vector<uint32_t> v(hv.begin() + long(gr.first), hv.begin() + long(gr.first + gr.count));
+            std::vector<uint32_t> q(hq.begin() + long(gr.first), hq.begin() + long(gr.first + gr.count));
+            std::vector<uint32_t> d(gr.count);
+            for (uint64_t i = 0; i < gr.count; ++i) d[i] = (*queries[q[i]].fdeg)[v[i]];
             if (int e = prepare_hubs(x, gr.k, v, q, d)) return e;
         }
         x.groups.push_back(gr);

@@ -9,12 +9,16 @@
 
 #include "kz_internal.h"
 
+// On failure the runtime's last-error state is cleared, so that a failed
+// allocation is not reported again by an unrelated later call.
 #define KZ_HIP(call)                                                                   \
     do {                                                                               \
         hipError_t _e = (call);                                                        \
-        if (_e != hipSuccess)                                                          \
+        if (_e != hipSuccess) {                                                        \
+            (void)hipGetLastError();                                                   \
             KZ_FAIL(KZ_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
                     __FILE__, __LINE__);                                               \
+        }                                                                              \
     } while (0)
 
 namespace kz {

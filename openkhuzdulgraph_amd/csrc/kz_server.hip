@@ -103,20 +103,43 @@ void release_prepared(kz_server_s* s, Prepared* p) {
 }
 
 // Device bytes a not yet prepared execution will take (upper estimates).
-uint64_t estimate_bytes(const DeviceGraph& d, int kind) {
-    if (kind == KZ_QUERY_CLIQUE)  // task lists and cursors; forward copy of a symmetric graph
-        return 8ull * d.n + (1u << 20) + (d.oriented || d.fwd_ready ? 0 : 8ull * (d.n + 1) + col_alloc_bytes(d.nnz));
+uint64_t estimate_bytes(const DeviceGraph& d, int kind, int k) {
+    if (kind == KZ_QUERY_CLIQUE) {
+        // task lists and cursors; for k >= 4 the hub roots' bitmap rows in
+        // HBM (d * ceil(d / 64) words each, out-degrees above the default
+        // 1024 bound; a symmetric graph's degrees bound its out-degrees);
+        // the forward copy of a symmetric graph
+        uint64_t hubs = 0;
+        if (k >= 4)
+            for (uint32_t x : d.h_deg)
+                if (x > 1024) hubs += uint64_t(x) * ((x + 63) / 64) * 8;
+        return 8ull * d.n + (1u << 20) + hubs +
+               (d.oriented || d.fwd_ready ? 0 : 8ull * (d.n + 1) + col_alloc_bytes(d.nnz));
+    }
     // relabelled copy, forward part, edge counters, 4-cycle slabs (at most
     // 2 workgroups per CU) and the 4-clique job
     return 8ull * (d.n + 1) * 3 + col_alloc_bytes(d.nnz) * 3 + 16ull * d.n * uint64_t(rt().num_cus) +
            8ull * d.n + (1u << 20);
 }
 
+// Device memory kept free next to what the server prepares.
+constexpr uint64_t kFreeMargin = uint64_t(1) << 30;
+
+uint64_t free_device_bytes() {
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return f;
+}
+
 // Frees idle prepared executions, least recently used first, until `need`
-// more bytes fit.  Returns whether they fit.
+// more bytes fit in the budget and in the device's free memory.  Returns
+// whether they fit.
 bool make_room(kz_server_s* s, uint64_t need) {
     const uint64_t B = budget_of(s);
-    while (s->in_use + need > B) {
+    while (s->in_use + need > B || (need && free_device_bytes() < need + kFreeMargin)) {
         Prepared* victim = nullptr;
         for (Prepared* p : s->prepared)
             if (!p->busy && (!victim || p->last_use < victim->last_use)) victim = p;
@@ -136,6 +159,22 @@ void finish(kz_server_s* s, SQuery* q, int status) {
     std::lock_guard<std::mutex> lk(s->mu);
     q->done = true;
     s->cv_done.notify_all();
+}
+
+// Releases every idle prepared execution except `keep`; returns whether any
+// was released.
+bool release_idle(kz_server_s* s, Prepared* keep) {
+    bool any = false;
+    for (bool again = true; again;) {
+        again = false;
+        for (Prepared* p : s->prepared)
+            if (p != keep && !p->busy) {
+                release_prepared(s, p);
+                any = again = true;
+                break;
+            }
+    }
+    return any;
 }
 
 bool any_busy(const kz_server_s* s) {
@@ -162,18 +201,41 @@ bool start_query(kz_server_s* s, SQuery* q, int slot) {
     DeviceGraph& d = q->g->g;
     Prepared* p = find_prepared(s, q->g, q->kind, q->k);
     if (!p) {
-        const uint64_t est = estimate_bytes(d, q->kind);
+        // an execution prepared before (by a direct call or an earlier
+        // server) is adopted with its real size
+        const bool cached = q->kind == KZ_QUERY_CLIQUE ? bool(d.clique_exec[q->k]) : bool(d.census);
+        if (cached) {
+            p = new Prepared{q->g, q->kind, q->k, 0, 0, 0};
+            s->prepared.push_back(p);
+            const uint64_t bytes =
+                q->kind == KZ_QUERY_CLIQUE ? static_cast<CliqueJob*>(d.clique_exec[q->k].get())->bytes : census_bytes(d);
+            if (!account(s, p, bytes)) {
+                if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
+                return !any_busy(s);
+            }
+        }
+    }
+    if (!p) {
+        const uint64_t est = estimate_bytes(d, q->kind, q->k);
         if (est > budget_of(s)) {
             finish(s, q, KZ_ERR_BUDGET);
             return true;
         }
-        if (!make_room(s, est)) return false;
+        if (!make_room(s, est)) {
+            // nothing running would free more: it cannot fit
+            if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
+            return !any_busy(s);
+        }
         p = new Prepared{q->g, q->kind, q->k, 0, 0, 0};
         s->prepared.push_back(p);
         if (q->kind == KZ_QUERY_CLIQUE) {
             const uint64_t fwd = (d.oriented || d.fwd_ready) ? 0 : 8ull * (d.n + 1) + col_alloc_bytes(d.nnz);
             CliqueJob* job = nullptr;
-            if (int e = graph_job(d, q->k, &job)) {
+            int e = graph_job(d, q->k, &job);
+            // out of device memory despite the estimate: release every idle
+            // preparation and try once more
+            if ((e == KZ_ERR_DEVICE || e == KZ_ERR_NOMEM) && release_idle(s, p)) e = graph_job(d, q->k, &job);
+            if (e) {
                 release_prepared(s, p);
                 finish(s, q, e);
                 return true;
@@ -381,7 +443,8 @@ int kz_server_stop(kz_server s) {
     for (auto st : s->streams) (void)hipStreamDestroy(st);
     for (auto ev : s->events) (void)hipEventDestroy(ev);
     (void)hipHostFree(s->pinned);
-    for (Prepared* p : s->prepared) delete p;
+    // the executions prepared under this server's budget are released
+    while (!s->prepared.empty()) release_prepared(s, s->prepared.back());
     for (auto& kv : s->live) delete kv.second;
     delete s;
     return KZ_OK;
