@@ -68,7 +68,7 @@ struct Ranked {
     uint32_t c4_grid = 0;
     uint32_t* cnt = nullptr;      // c4_grid * n counters
     uint32_t* touched = nullptr;  // c4_grid * n touched lists
-    uint32_t* cursor = nullptr;   // 4 task cursors
+    uint32_t* cursor = nullptr;   // 4 task cursors, kCursorStride words apart
     unsigned long long* sums = nullptr;  // 8 accumulators
     uint64_t bytes = 0;           // device memory held (with the 4-clique job)
     CliqueJob k4;
@@ -157,8 +157,9 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
     __shared__ uint32_t sTask, sTmp[T / 64];
     const RootSet rs{hk, hv, H - 1, hshift, fb, fshift};
+    TaskChunk chunk(ntasks, gridDim.x, T == 64 ? 2 : 8);
     while (true) {
-        if (threadIdx.x == 0) sTask = atomicAdd(cursor, 1u);
+        if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
         __syncthreads();
         const uint32_t t = sTask;
         if (t >= ntasks) break;
@@ -191,8 +192,7 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
         };
         stream_groups(fcol, rb, rp, nr, G, threadIdx.x, T, rs, L[d - 1], hit);
         // skewed rows: each wave searches the candidates of its own rows
-        search_skewed(frowptr, fcol, rb, L, d, 0, nr, threadIdx.x >> 6, T / 64, hit,
-                      [&](uint32_t r) { return uint32_t(ldg(frowptr, L[r] + 1) - (rb[r] & kStartMask)); });
+        search_skewed(fcol, rb, L, d, 0, nr, threadIdx.x >> 6, T / 64, hit);
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < d; i += T) troot[ub + i] = cnt[i];
         __syncthreads();
@@ -243,8 +243,9 @@ __global__ __launch_bounds__(256) void census_c4_kernel(const uint64_t* __restri
     const uint32_t nw = blockDim.x >> 6;
     unsigned long long acc = 0;
     if (threadIdx.x == 0) sTouched = 0;
+    TaskChunk chunk(n, gridDim.x, 8);
     while (true) {
-        if (threadIdx.x == 0) sTask = atomicAdd(cursor, 1u);
+        if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
         __syncthreads();
         const uint32_t t = sTask;
         if (t >= n) break;
@@ -418,7 +419,7 @@ static int prepare_passes(Ranked& R) {
     if (!all.empty()) KZ_HIP(hipMemcpy(R.tasks, all.data(), 4 * all.size(), hipMemcpyHostToDevice));
     KZ_HIP(hipMalloc(&R.troot, 4 * std::max<uint64_t>(1, R.fnnz)));
     KZ_HIP(hipMalloc(&R.tmid, 4 * std::max<uint64_t>(1, R.fnnz)));
-    KZ_HIP(hipMalloc(&R.cursor, 4 * 4));
+    KZ_HIP(hipMalloc(&R.cursor, 4 * 4 * kCursorStride));
     KZ_HIP(hipMalloc(&R.sums, 8 * 8));
     // 4-cycle workgroups: two per CU, each with 8 bytes per vertex of slab,
     // fewer when the slabs would take more than a quarter of free memory
@@ -463,7 +464,7 @@ static int run_census(Ranked& R, uint64_t* sums, double* device_ms) {
     KZ_HIP(hipEventCreate(&e0));
     KZ_HIP(hipEventCreate(&e1));
     KZ_HIP(hipEventRecord(e0, s));
-    hipLaunchKernelGGL(fill_small_kernel, dim3(1), dim3(64), 0, s, R.cursor, 4, R.sums, 8);
+    hipLaunchKernelGGL(fill_small_kernel, dim3(1), dim3(64), 0, s, R.cursor, int(4 * kCursorStride), R.sums, 8);
     const uint64_t fz = std::max<uint64_t>(1, R.fnnz);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(2048), dim3(256), 0, s, R.troot, fz);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(2048), dim3(256), 0, s, R.tmid, fz);
@@ -491,12 +492,12 @@ static int run_census(Ranked& R, uint64_t* sums, double* device_ms) {
         uint32_t fw, fs;
         filter_geometry(R.large_dmax, &fw, &fs);
         hipLaunchKernelGGL(census_tri_kernel<256>, dim3(grid), dim3(256), lds, s, R.frowptr, R.fcol,
-                           R.tasks + R.nsmall, R.nlarge, R.cursor + 1, R.troot, R.tmid, R.large_dmax, H, hs, fw, fs);
+                           R.tasks + R.nsmall, R.nlarge, R.cursor + kCursorStride, R.troot, R.tmid, R.large_dmax, H, hs, fw, fs);
     }
     hipLaunchKernelGGL(census_sums_kernel, dim3(std::max(1, rt_.num_cus * 4)), dim3(256), 0, s, R.frowptr, R.fcol,
                        R.deg, R.n, R.troot, R.tmid, R.sums);
     if (R.n)
-        hipLaunchKernelGGL(census_c4_kernel, dim3(R.c4_grid), dim3(256), 0, s, R.rowptr, R.col, R.n, R.cursor + 2,
+        hipLaunchKernelGGL(census_c4_kernel, dim3(R.c4_grid), dim3(256), 0, s, R.rowptr, R.col, R.n, R.cursor + 2 * kCursorStride,
                            R.cnt, R.touched, R.sums);
     KZ_HIP(hipGetLastError());
     KZ_HIP(hipEventRecord(e1, s));

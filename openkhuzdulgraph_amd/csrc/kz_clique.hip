@@ -65,9 +65,10 @@ struct HubMeta {
 // One task per wave, |N+(u)| <= 64.  LDS per wave: L[64], 128-slot hash,
 // 64 bitmap rows (one u64 each) and the row descriptors.  Tasks are small,
 // so the chain cursor -> root -> list bounds of the NEXT task is issued
-// while the current one streams and counts.
+// while the current one streams and counts.  Register budget: 5 waves per
+// SIMD (at most 96 VGPRs, no spills).
 template <int K>
-__global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __restrict__ queries,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void clique_wave_kernel(const QueryRef* __restrict__ queries,
                                                           const uint32_t* __restrict__ task_v,
                                                           const uint32_t* __restrict__ task_q, uint32_t ntasks,
                                                           uint32_t* __restrict__ cursor,
@@ -92,11 +93,9 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
     uint32_t* rp = sRp[w];
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
-    auto fetch = [&]() {
-        uint32_t v = 0;
-        if (lane == 0) v = atomicAdd(cursor, 1u);
-        return __shfl(v, 0);
-    };
+    const uint32_t workers = gridDim.x * WPB;
+    TaskChunk chunk(ntasks, workers, 2);
+    auto fetch = [&]() { return wave_next_task(chunk, cursor); };
     // current task: index, query, list start and length, graph arrays
     uint32_t t = fetch(), q = 0, d = 0;
     uint64_t ub = 0;
@@ -134,7 +133,7 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
         rp[lane] = excl;
         wave_lds_sync();
         if (lane < d) root_insert(rs, x, uint16_t(lane));
-        // next task's query and root (the cursor atomic has returned by now)
+        // next task's query and root (a cursor atomic has returned by now)
         uint32_t qn = q, un = 0;
         if (tn < ntasks) {
             qn = task_q ? task_q[tn] : 0u;
@@ -142,7 +141,7 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
         }
         wave_lds_sync();
         const uint32_t lmax = L[d - 1];
-        stream_groups(col, rb, rp, d - 1, G, lane, 64, rs, lmax,
+        stream_groups<kGroups, (K >= 4)>(col, rb, rp, d - 1, G, lane, 64, rs, lmax,
                       [&](uint32_t r, uint32_t j, uint64_t) {
                           if (K == 3) acc += 1;
                           else atomicOr(&rows[r], 1ull << j);
@@ -160,16 +159,12 @@ __global__ __launch_bounds__(256) void clique_wave_kernel(const QueryRef* __rest
             ubn = ldg(rowptr_n, un);
             dn = uint32_t(ldg(rowptr_n, un + 1) - ubn);
         }
-        // skewed rows: their (row, candidate) pairs spread over the lanes;
-        // lane r holds row r's list length
+        // skewed rows: their (row, candidate) pairs spread over the lanes
         if (__ballot(skewed))
-            search_skewed(
-                rowptr, col, rb, L, d, 0, d - 1, 0, 1,
-                [&](uint32_t r, uint32_t j, uint64_t) {
-                    if (K == 3) acc += 1;
-                    else atomicOr(&rows[r], 1ull << j);
-                },
-                [&](uint32_t) { return len; });
+            search_skewed(col, rb, L, d, 0, d - 1, 0, 1, [&](uint32_t r, uint32_t j, uint64_t) {
+                if (K == 3) acc += 1;
+                else atomicOr(&rows[r], 1ull << j);
+            });
         wave_lds_sync();
         if (K >= 4 && lane < d) {
             const unsigned long long ri = rows[lane];
@@ -266,27 +261,28 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
     }
     __syncthreads();
     const uint32_t G = block_exclusive_scan(rp, nr + 1, tmp);
-    stream_groups(col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
+    stream_groups<kGroups, (K >= 4)>(col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
                   [&](uint32_t r, uint32_t j, uint64_t) {
                       if (K == 3) acc += 1;
                       else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
                   });
     // skewed rows: each wave searches the candidates of its own rows
-    search_skewed(rowptr, col, rb, L, d, r0, nr, threadIdx.x >> 6, blockDim.x >> 6,
-                  [&](uint32_t r, uint32_t j, uint64_t) {
-                      if (K == 3) acc += 1;
-                      else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
-                  },
-                  [&](uint32_t r) { return uint32_t(ldg(rowptr, L[r0 + r] + 1) - (rb[r] & kStartMask)); });
+    search_skewed(col, rb, L, d, r0, nr, threadIdx.x >> 6, blockDim.x >> 6, [&](uint32_t r, uint32_t j, uint64_t) {
+        if (K == 3) acc += 1;
+        else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
+    });
     __syncthreads();
 }
 
 // ----------------------------------------------------------- block tasks --
 // One task per workgroup, rows in LDS.  Dynamic LDS: [rows Dmax*W u64 (k>=4)]
 // [row descriptors Dmax u64][L Dmax u32][group prefix Dmax+1 u32]
-// [filter FW u32][hash keys H u32][hash vals H u16].
+// [filter FW u32][hash keys H u32][hash vals H u16].  Register budget: 6
+// waves per SIMD (at most 80 VGPRs), the occupancy the LDS of the middle
+// buckets allows; the latency-bound count phase loses more to fewer waves
+// than the stream gains from more registers.
 template <int K>
-__global__ __launch_bounds__(1024) void clique_block_kernel(const QueryRef* __restrict__ queries,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void clique_block_kernel(const QueryRef* __restrict__ queries,
                                                            const uint32_t* __restrict__ task_v,
                                                            const uint32_t* __restrict__ task_q, uint32_t ntasks,
                                                            uint32_t* __restrict__ cursor,
@@ -319,8 +315,9 @@ __global__ __launch_bounds__(1024) void clique_block_kernel(const QueryRef* __re
     uint32_t qcur = 0xFFFFFFFFu;
     const uint64_t* rowptr = nullptr;
     const uint32_t* col = nullptr;
+    TaskChunk chunk(ntasks, gridDim.x, 8);
     while (true) {
-        if (threadIdx.x == 0) sTask = atomicAdd(cursor, 1u);
+        if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
         __syncthreads();
         const uint32_t t = sTask;
         if (t >= ntasks) break;
@@ -387,8 +384,9 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
     const uint64_t* rowptr = nullptr;
     const uint32_t* col = nullptr;
     RootSet rs{hk, hv, 0, 0, fb, 0};
+    TaskChunk chunk(nitems, gridDim.x, 8);
     while (true) {
-        if (threadIdx.x == 0) sItem = atomicAdd(cursor, 1u);
+        if (threadIdx.x == 0) sItem = thread_next_task(chunk, cursor);
         __syncthreads();
         const uint32_t it = sItem;
         if (it >= nitems) break;
@@ -447,10 +445,9 @@ __global__ __launch_bounds__(256) void hub_count_kernel(const HubMeta* __restric
     const uint32_t lane = threadIdx.x & 63;
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
+    TaskChunk chunk(nrows, gridDim.x * (blockDim.x >> 6), 8);
     while (true) {
-        uint32_t r = 0;
-        if (lane == 0) r = atomicAdd(cursor, 1u);
-        r = __shfl(r, 0);
+        const uint32_t r = wave_next_task(chunk, cursor);
         if (r >= nrows) break;
         const uint32_t h = row_of(row_prefix, nhubs, r);
         const HubMeta hm = hubs[h];
@@ -616,7 +613,7 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     if (K >= 4) {
         const uint32_t cgrid = std::min<uint32_t>((hp.nrows + 3) / 4, uint32_t(R.num_cus) * 8);
         hipLaunchKernelGGL(hub_count_kernel<K>, dim3(cgrid), dim3(256), 0, s, meta, hp.nhubs, x.hub_rows[K], hp.nrows,
-                           cur + 1, x.d_result, x.slabs[K]);
+                           cur + kCursorStride, x.d_result, x.slabs[K]);
     }
     return KZ_OK;
 }
@@ -747,7 +744,8 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         const uint32_t dfirst = (*queries[hq[gr.first]].fdeg)[hv[gr.first]];  // the largest in the group
         gr.dmax = gr.bucket == 0 ? 64 : (gr.bucket == kNumBuckets - 1 ? dfirst : cfg.hi[gr.bucket]);
         gr.cursor = cursor;
-        cursor += (gr.bucket == kNumBuckets - 1) ? 2 : 1;  // hubs: rows + count cursors
+        // one cursor line per kernel (hubs: rows and count kernels)
+        cursor += int(kCursorStride) * ((gr.bucket == kNumBuckets - 1) ? 2 : 1);
         if (gr.bucket == kNumBuckets - 1) {
             std::vector<uint32_t> v(hv.begin() + long(gr.first), hv.begin() + long(gr.first + gr.count));
             std::vector<uint32_t> q(hq.begin() + long(gr.first), hq.begin() + long(gr.first + gr.count));

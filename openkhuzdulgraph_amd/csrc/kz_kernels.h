@@ -21,9 +21,11 @@ constexpr int kStaticLdsReserve = 1024;      // static LDS headroom of the block
 // Open-addressing LDS hash, vertex id -> local index.  Multiplicative
 // hashing on the top bits; linear probing; the table is at least twice the
 // key count so probes stay short.
+__device__ __forceinline__ uint32_t hash_slot(uint32_t x, uint32_t shift) { return (x * 2654435761u) >> shift; }
+
 __device__ __forceinline__ void hash_insert(uint32_t* keys, uint16_t* vals, uint32_t mask, uint32_t shift,
                                             uint32_t x, uint16_t idx) {
-    uint32_t h = (x * 2654435761u) >> shift;
+    uint32_t h = hash_slot(x, shift);
     while (true) {
         const uint32_t old = atomicCAS(&keys[h], kEmpty, x);
         if (old == kEmpty) {
@@ -34,15 +36,68 @@ __device__ __forceinline__ void hash_insert(uint32_t* keys, uint16_t* vals, uint
     }
 }
 
-__device__ __forceinline__ uint32_t hash_find(const uint32_t* keys, const uint16_t* vals, uint32_t mask,
-                                              uint32_t shift, uint32_t x) {
-    uint32_t h = (x * 2654435761u) >> shift;
+// Continues a probe for x at slot h.
+__device__ __forceinline__ uint32_t hash_probe(const uint32_t* keys, const uint16_t* vals, uint32_t mask, uint32_t h,
+                                               uint32_t x) {
     while (true) {
         const uint32_t k = keys[h];
         if (k == x) return vals[h];
         if (k == kEmpty) return kNotFound;
         h = (h + 1) & mask;
     }
+}
+
+__device__ __forceinline__ uint32_t hash_find(const uint32_t* keys, const uint16_t* vals, uint32_t mask,
+                                              uint32_t shift, uint32_t x) {
+    return hash_probe(keys, vals, mask, hash_slot(x, shift), x);
+}
+
+// Persistent kernels pull tasks from a device cursor in chunks: one atomic
+// reserves `size` consecutive tasks.  Tasks come largest first, so the size
+// ramps up with the tasks already taken (at most base / (ramp * workers))
+// and shrinks again with the tasks left (at most an eighth of the caller's
+// share of them), within [1, kMaxTaskChunk]: the heavy first tasks still go
+// one at a time, the many light ones in chunks.  A slow ramp (8) suits
+// kernels whose task costs vary by orders of magnitude (workgroup tasks),
+// a fast one (2) kernels of small, similar tasks (wave tasks).  Each cursor
+// is the first u32 of its own kCursorStride-word (256-byte) line: atomics on
+// one line are serialised at one L2 channel, on the order of 10^8 per
+// second, which bounds a kernel of short tasks taking one per atomic.
+constexpr uint32_t kCursorStride = 64;
+constexpr uint32_t kMaxTaskChunk = 32;
+
+__device__ __forceinline__ uint32_t task_chunk(uint32_t base, uint32_t ntasks, uint32_t workers, uint32_t ramp) {
+    const uint32_t left = base < ntasks ? ntasks - base : 0u;
+    const uint32_t a = left / (8u * workers), b = base / (ramp * workers);
+    const uint32_t c = a < b ? a : b;
+    return c < 1u ? 1u : (c > kMaxTaskChunk ? kMaxTaskChunk : c);
+}
+
+struct TaskChunk {
+    uint32_t next, end, size, ntasks, workers, ramp;
+    __device__ TaskChunk(uint32_t n, uint32_t w, uint32_t r) : next(0), end(0), size(1), ntasks(n), workers(w), ramp(r) {}
+    __device__ void take(uint32_t base) {
+        next = base;
+        end = base + size;
+        size = task_chunk(end, ntasks, workers, ramp);
+    }
+};
+
+// Next task of a wave (call with every lane; the result is wave-uniform,
+// >= ntasks when none is left).
+__device__ __forceinline__ uint32_t wave_next_task(TaskChunk& c, uint32_t* cursor) {
+    if (c.next == c.end) {
+        uint32_t v = 0;
+        if ((threadIdx.x & 63) == 0) v = atomicAdd(cursor, c.size);
+        c.take(__shfl(v, 0));
+    }
+    return c.next++;
+}
+
+// Next task of a workgroup, called by one thread only.
+__device__ __forceinline__ uint32_t thread_next_task(TaskChunk& c, uint32_t* cursor) {
+    if (c.next == c.end) c.take(atomicAdd(cursor, c.size));
+    return c.next++;
 }
 
 // Load through a pointer known to point at device (HBM) memory.  Graph

@@ -75,28 +75,32 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t* a, uint32_t n
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Groups of 4 entries each lane keeps in flight per streaming iteration.
-constexpr int kGroups = 4;
+constexpr int kGroups = 2;
 
 // Descriptor of one row, the neighbour list N+(v) = col[vb, ve), with
-// `cand` candidates left after it: bits 0..60 hold vb, bit 61 marks a
-// "skewed" row and bits 62..63 hold ve & 3.  A streamed row is read as
-// aligned 16-byte groups from vb & ~3 (*groups of them); the first group
-// starts at entry vb & 3 and the last one ends at ve & 3 (0: full).  A
-// skewed row, whose list is more than kSkewFactor times longer than its
-// candidates, is not streamed (*groups = 0): its candidates are
-// binary-searched in the list.  *len gets ve - vb.  Returns whether the row
-// is skewed.  Column arrays are allocated with col_alloc_bytes, so the last
-// group stays inside the allocation.
+// `cand` candidates left after it.  A streamed row keeps vb in bits 0..60 and
+// ve & 3 in bits 62..63; it is read as aligned 16-byte groups from vb & ~3
+// (*groups of them), the first group starting at entry vb & 3 and the last
+// one ending at ve & 3 (0: full).  A skewed row, whose list is more than
+// kSkewFactor times longer than its candidates, is not streamed
+// (*groups = 0): its candidates are binary-searched in the list; bit 61 marks
+// it, bits 0..39 hold vb and bits 40..60 the length (lists too long or too
+// far into col for those fields are streamed).  *len gets ve - vb.  Returns
+// whether the row is skewed.  Column arrays are allocated with
+// col_alloc_bytes, so the last group stays inside the allocation.
 constexpr uint64_t kSkewBit = 1ull << 61;
 constexpr uint64_t kStartMask = kSkewBit - 1;
+constexpr int kSkewLenShift = 40;
+constexpr uint64_t kSkewStartMask = (1ull << kSkewLenShift) - 1;
+constexpr uint64_t kSkewMaxLen = (1ull << (61 - kSkewLenShift)) - 1;
 constexpr uint64_t kSkewFactor = 4;
 
 __device__ __forceinline__ bool row_desc(const uint64_t* rowptr, uint32_t v, uint32_t cand, uint64_t* desc,
                                          uint32_t* groups, uint32_t* len) {
     const uint64_t vb = ldg(rowptr, v), ve = ldg(rowptr, v + 1);
     *len = uint32_t(ve - vb);
-    if (ve - vb > kSkewFactor * cand) {
-        *desc = vb | kSkewBit;
+    if (ve - vb > kSkewFactor * cand && ve - vb <= kSkewMaxLen && vb <= kSkewStartMask) {
+        *desc = vb | ((ve - vb) << kSkewLenShift) | kSkewBit;
         *groups = 0;
         return true;
     }
@@ -154,61 +158,137 @@ __device__ __forceinline__ uint32_t row_in(const uint32_t* p, uint32_t lo, uint3
     return lo;
 }
 
+// Largest i in [lo, hi) with p[i] <= f, given p[lo] <= f (p non-decreasing);
+// wave-uniform arguments and result.  Each round reads 64 evenly spaced
+// entries, one per lane, and narrows the range 64-fold with one ballot, so
+// ranges up to 4096 take two LDS round trips.
+__device__ __forceinline__ uint32_t wave_search(const uint32_t* p, uint32_t lo, uint32_t hi, uint32_t f) {
+    const uint32_t lane = threadIdx.x & 63;
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) >> 6;
+        const uint32_t i = lo + lane * step;
+        const unsigned long long b = __ballot(i < hi && p[i] <= f);
+        lo += (uint32_t(__popcll(b)) - 1) * step;
+        hi = hi < lo + step ? hi : lo + step;
+    }
+    return lo;
+}
+
 // Streams groups [0, G) of rows [0, nr) described by desc and gp (exclusive
 // prefix of the group counts, nr + 1 entries) over threads tid = 0 .. T-1
 // (whole waves).  Every entry x <= lmax found in the root's set at index j
-// is a hit: hit(r, j, p) with p the entry's position in col.  Each wave
-// takes 64 * kGroups consecutive groups per iteration: two wave-uniform
-// searches bound their rows, each lane searches only that window, and each
-// lane issues kGroups independent 16-byte loads (consecutive across the
-// wave) before it probes.
-template <class Hit>
+// is a hit: hit(r, j, p) with p the entry's position in col (j is only
+// looked up when kIndex).
+//
+// Each wave takes a contiguous share of the groups and walks it NG * 64
+// groups per iteration, lane l taking groups l, l + 64, ...  The iteration's
+// first row is the previous iteration's last, its last row comes from one
+// wave_search, and each lane finds the rows of its NG groups with NG binary
+// searches run in lockstep.  After the NG 16-byte loads, the 4 * NG entries
+// go through the root set in batched phases, each issuing all its LDS reads
+// before it waits: filter words, then the first two hash slots of the
+// entries that pass the filter, then the indices of the keys found.  Only
+// keys displaced further (rare at a load factor of at most 1/2) probe on.
+template <int NG = kGroups, bool kIndex = true, class Hit>
 __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigned long long* desc, const uint32_t* gp,
                                               uint32_t nr, uint32_t G, uint32_t tid, uint32_t T, const RootSet& rs,
                                               uint32_t lmax, Hit&& hit) {
+    constexpr int NE = 4 * NG;
     const uint32_t lane = tid & 63;
-    const uint32_t wofs = (tid - lane) * kGroups;
-    for (uint32_t cbase = 0; cbase < G; cbase += kGroups * T) {
-        const uint32_t g0 = cbase + wofs;
-        if (g0 >= G) continue;
-        const uint32_t g1 = (g0 + 64 * kGroups < G ? g0 + 64 * kGroups : G) - 1;
-        const uint32_t rlo = row_of(gp, nr, g0);
-        const uint32_t rhi = row_in(gp, rlo, nr, g1) + 1;
-        u32x4 y[kGroups];
-        uint32_t rr[kGroups], lo[kGroups], hi[kGroups];
-        uint64_t at[kGroups];
+    const uint32_t nw = T >> 6;
+    const uint32_t per = ((G + nw - 1) / nw + 63) & ~63u;
+    const uint32_t wb = (tid >> 6) * per;
+    const uint32_t wbeg = wb < G ? wb : G;
+    const uint32_t wend = wbeg + per < G ? wbeg + per : G;
+    if (wbeg >= wend) return;
+    uint32_t rlo = wave_search(gp, 0, nr, wbeg);
+    for (uint32_t g0 = wbeg; g0 < wend; g0 += 64 * NG) {
+        const uint32_t g1 = (g0 + 64 * NG < wend ? g0 + 64 * NG : wend) - 1;
+        const uint32_t rhi = wave_search(gp, rlo, nr, g1) + 1;
+        // rows of this lane's groups (past g1: the group g1, masked below)
+        uint32_t gg[NG], rr[NG];
 #pragma unroll
-        for (int k = 0; k < kGroups; ++k) {
+        for (int k = 0; k < NG; ++k) {
             const uint32_t g = g0 + uint32_t(k) * 64 + lane;
-            y[k] = u32x4{0u, 0u, 0u, 0u};
-            rr[k] = 0;
-            lo[k] = 4;
-            hi[k] = 0;
-            at[k] = 0;
-            if (g <= g1) {
-                const uint32_t r = row_in(gp, rlo, rhi, g);
-                const uint32_t q = g - gp[r];
-                const uint64_t ds = desc[r];
-                const uint64_t vb = ds & kStartMask;
-                const uint32_t e3 = uint32_t(ds >> 62);
-                rr[k] = r;
-                lo[k] = q ? 0u : uint32_t(vb & 3);
-                hi[k] = (g + 1 == gp[r + 1] && e3) ? e3 : 4u;
-                at[k] = (vb & ~uint64_t(3)) + 4ull * q;
-                y[k] = ldg(reinterpret_cast<const u32x4*>(col + (vb & ~uint64_t(3))), q);
+            gg[k] = g < g1 ? g : g1;
+            rr[k] = rlo;
+        }
+        for (uint32_t len = rhi - rlo; len > 1;) {
+            const uint32_t half = len >> 1;
+#pragma unroll
+            for (int k = 0; k < NG; ++k) {
+                const uint32_t m = rr[k] + half;
+                rr[k] = gp[m] <= gg[k] ? m : rr[k];
+            }
+            len -= half;
+        }
+        u32x4 y[NG];
+        uint32_t lo[NG], hi[NG];
+        uint64_t at[NG];
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+            const uint32_t r = rr[k];
+            const uint32_t p0 = gp[r], p1 = gp[r + 1];
+            const uint64_t ds = desc[r];
+            const uint64_t vb = ds & kStartMask;
+            const uint32_t e3 = uint32_t(ds >> 62);
+            const uint32_t q = gg[k] - p0;
+            const bool live = g0 + uint32_t(k) * 64 + lane <= g1;
+            lo[k] = !live ? 4u : q ? 0u : uint32_t(vb & 3);
+            hi[k] = (gg[k] + 1 == p1 && e3) ? e3 : 4u;
+            at[k] = (vb & ~uint64_t(3)) + 4ull * q;
+            y[k] = ldg(reinterpret_cast<const u32x4*>(col + (vb & ~uint64_t(3))), q);
+        }
+        // filter
+        uint32_t pass = 0;
+        {
+            uint32_t fw[NE];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) fw[e] = rs.fb[filter_bit(y[e >> 2][e & 3], rs.fshift) >> 5];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                const uint32_t x = y[e >> 2][e & 3];
+                const uint32_t c = uint32_t(e & 3);
+                const bool ok = c >= lo[e >> 2] && c < hi[e >> 2] && x <= lmax &&
+                                ((fw[e] >> (filter_bit(x, rs.fshift) & 31)) & 1u);
+                pass |= uint32_t(ok) << e;
             }
         }
+        // first two hash slots (lanes without a candidate read slot 0: a
+        // broadcast)
+        uint32_t found = 0, cont = 0;
+        uint32_t slot[NE];
 #pragma unroll
-        for (int k = 0; k < kGroups; ++k) {
+        for (int e = 0; e < NE; ++e) {
+            const uint32_t x = y[e >> 2][e & 3];
+            const bool pe = (pass >> e) & 1u;
+            const uint32_t h0 = pe ? hash_slot(x, rs.hshift) : 0u;
+            const uint32_t h1 = pe ? (h0 + 1) & rs.hmask : 0u;
+            const uint32_t k0 = rs.hk[h0], k1 = rs.hk[h1];
+            const bool f0 = pe && k0 == x;
+            const bool past0 = pe && k0 != x && k0 != kEmpty;
+            const bool f1 = past0 && k1 == x;
+            found |= uint32_t(f0 || f1) << e;
+            cont |= uint32_t(past0 && k1 != x && k1 != kEmpty) << e;
+            slot[e] = f0 ? h0 : h1;
+        }
+        uint32_t jj[NE];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const uint32_t x = y[k][c];
-                if (uint32_t(c) >= lo[k] && uint32_t(c) < hi[k] && x <= lmax) {
-                    const uint32_t j = root_find(rs, x);
-                    if (j != kNotFound) hit(rr[k], j, at[k] + uint32_t(c));
+        for (int e = 0; e < NE; ++e) jj[e] = kIndex ? rs.hv[((found >> e) & 1u) ? slot[e] : 0u] : 0u;
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+            if ((found >> e) & 1u) hit(rr[e >> 2], jj[e], at[e >> 2] + uint32_t(e & 3));
+        if (cont) {
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                if ((cont >> e) & 1u) {
+                    const uint32_t x = y[e >> 2][e & 3];
+                    const uint32_t j = hash_probe(rs.hk, rs.hv, rs.hmask, (slot[e] + 1) & rs.hmask, x);
+                    if (j != kNotFound) hit(rr[e >> 2], j, at[e >> 2] + uint32_t(e & 3));
                 }
             }
         }
+        rlo = rhi - 1;
     }
 }
 
@@ -219,18 +299,18 @@ __device__ __forceinline__ uint32_t find_global(const uint32_t* __restrict__ a, 
 }
 
 // Candidates of the skewed rows among rows [0, nr) (list indices r0 + r of
-// L, d ids; candidates of row r are L[r0 + r + 1 .. d)), binary-searched in
-// the rows' lists.  Wave `wave` of nw takes rows r = wave, wave + nw, ...,
-// 64 at a time, and spreads their (row, candidate) pairs over its lanes, so
-// the searches of 64 pairs are in flight together; no workgroup barrier, so
-// it overlaps other waves' streaming.  hit(r, j, p): candidate j found at
-// position p of col.  row_len(r), called by the lane that takes row r, gives
-// the row's list length.  Call with every lane of the wave.
-template <class Hit, class Len>
-__device__ __forceinline__ void search_skewed(const uint64_t* rowptr, const uint32_t* col,
-                                              const unsigned long long* desc, const uint32_t* L, uint32_t d,
-                                              uint32_t r0, uint32_t nr, uint32_t wave, uint32_t nw, Hit&& hit,
-                                              Len&& row_len) {
+// L, d ids; candidates of row r are L[r0 + r + 1 .. d)), searched in the
+// rows' lists.  Wave `wave` of nw takes rows r = wave, wave + nw, ..., 64 at a
+// time, and spreads their (row, candidate) pairs over its lanes, NP pairs per
+// lane: the NP binary searches of a lane advance in lockstep, so each step
+// has 64 * NP independent loads in flight (a search is a chain of dependent
+// HBM/L2 loads, the latency is what bounds it).  No workgroup barrier, so it
+// overlaps other waves' streaming.  hit(r, j, p): candidate j found at
+// position p of col.  Call with every lane of the wave.
+template <int NP = 1, class Hit>
+__device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigned long long* desc, const uint32_t* L,
+                                              uint32_t d, uint32_t r0, uint32_t nr, uint32_t wave, uint32_t nw,
+                                              Hit&& hit) {
     const uint32_t lane = threadIdx.x & 63;
     for (uint32_t rbase = wave; rbase < nr; rbase += 64 * nw) {
         const uint32_t r = rbase + lane * nw;
@@ -239,31 +319,57 @@ __device__ __forceinline__ void search_skewed(const uint64_t* rowptr, const uint
         if (r < nr) {
             const uint64_t ds = desc[r];
             if (ds & kSkewBit) {
-                b = ds & kStartMask;
+                b = ds & kSkewStartMask;
+                n = uint32_t((ds & kStartMask) >> kSkewLenShift);
                 cnt = d - 1 - (r0 + r);
-                n = row_len(r);
             }
         }
         uint32_t excl;
         const uint32_t P = wave_exclusive_scan(cnt, &excl);
-        for (uint32_t pbase = 0; pbase < P; pbase += 64) {
-            const uint32_t p = pbase + lane;
-            // owning lane: the largest l with excl(l) <= p
-            uint32_t l = 0;
+        for (uint32_t pbase = 0; pbase < P; pbase += 64 * NP) {
+            uint32_t rr[NP], jj[NP], xx[NP], at[NP], len[NP], v[NP];
+            uint64_t bb[NP];
 #pragma unroll
-            for (uint32_t step = 32; step; step >>= 1) {
-                const uint32_t e = __shfl(excl, int(l + step));
-                if (e <= p) l += step;
+            for (int k = 0; k < NP; ++k) {
+                const uint32_t p = pbase + uint32_t(k) * 64 + lane;
+                // owning lane: the largest l with excl(l) <= p
+                uint32_t l = 0;
+#pragma unroll
+                for (uint32_t step = 32; step; step >>= 1) {
+                    const uint32_t e = __shfl(excl, int(l + step));
+                    if (e <= p) l += step;
+                }
+                const bool live = p < P;
+                rr[k] = __shfl(r, int(l));
+                bb[k] = __shfl(b, int(l));
+                const uint32_t nn = __shfl(n, int(l));
+                jj[k] = r0 + rr[k] + 1 + (p - __shfl(excl, int(l)));
+                xx[k] = live ? L[jj[k]] : 0u;
+                len[k] = live ? nn : 0u;
+                at[k] = 0;
             }
-            const uint32_t rr = __shfl(r, int(l));
-            const uint64_t bb = __shfl(b, int(l));
-            const uint32_t nn = __shfl(n, int(l));
-            const uint32_t e0 = __shfl(excl, int(l));
-            if (p < P) {
-                const uint32_t j = r0 + rr + 1 + (p - e0);
-                const uint32_t pos = find_global(col + bb, nn, L[j]);
-                if (pos < nn) hit(rr, j, bb + pos);
+            // largest i with a[i] <= x (a = col + bb); v tracks a[at]
+#pragma unroll
+            for (int k = 0; k < NP; ++k) v[k] = ldg(col, bb[k]);
+            while (true) {
+                bool more = false;
+#pragma unroll
+                for (int k = 0; k < NP; ++k) more |= len[k] > 1;
+                if (!__ballot(more)) break;
+#pragma unroll
+                for (int k = 0; k < NP; ++k) {
+                    const uint32_t half = len[k] >> 1;
+                    const uint32_t t = ldg(col, bb[k] + at[k] + half);
+                    if (half && t <= xx[k]) {
+                        at[k] += half;
+                        v[k] = t;
+                    }
+                    len[k] -= half;
+                }
             }
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                if (pbase + uint32_t(k) * 64 + lane < P && v[k] == xx[k]) hit(rr[k], jj[k], bb[k] + at[k]);
         }
     }
 }

@@ -29,8 +29,9 @@ __global__ __launch_bounds__(256) void workload_kernel(const uint64_t* __restric
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     unsigned long long a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    TaskChunk chunk(n, gridDim.x, 8);
     while (true) {
-        if (threadIdx.x == 0) sTask = atomicAdd(cursor, 1u);
+        if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
         __syncthreads();
         const uint32_t u = sTask;
         if (u >= n) break;
