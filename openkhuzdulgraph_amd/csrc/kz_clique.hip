@@ -240,6 +240,68 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
     }
 }
 
+// ------------------------------------------------------- MFMA 4-cliques --
+// With A the d x d 0/1 matrix of the root's DAG (A[i][j] = bit j of row i,
+// j > i), the 4-cliques at the root are sum_{i<j} A[i][j] * |row_i & row_j|
+// = sum_{i,j} A[i][j] * (A A^T)[i][j]: an int8 GEMM masked by A.  The
+// workgroup's waves take 32 x 32 tiles (I, J), J >= I, of A A^T, one
+// v_mfma_i32_32x32x32_i8 per 32 columns; A A^T[i][j] only gathers columns
+// l > j >= 32 J, so column blocks start at J.  Operands come straight from
+// the LDS bitmap: lane l holds row (or column) l % 32 of the tile and the 16
+// bits (l / 32) of the 32-column block, expanded to 16 bytes of 0/1; the
+// k order inside a block is the same for both operands, so any order works.
+// Accumulator r of lane l holds tile row (r / 4) * 8 + (l / 32) * 4 + r % 4,
+// column l % 32.
+typedef int kz_v4i __attribute__((ext_vector_type(4)));
+typedef int kz_v16i __attribute__((ext_vector_type(16)));
+
+// 16 bits as 16 bytes of 0 or 1 (bit t in byte t): a 24-bit multiply
+// spreads each nibble's bits 7 apart, the mask keeps one per byte.
+__device__ __forceinline__ kz_v4i bits_to_bytes(uint32_t f) {
+    kz_v4i r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[q] = int(__umul24((f >> (4 * q)) & 0xFu, 0x00204081u) & 0x01010101u);
+    return r;
+}
+
+__device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows, uint32_t ws, uint32_t d,
+                                                 uint32_t wave, uint32_t nwaves, unsigned long long& acc) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nb = (d + 31) >> 5;
+    uint32_t tile = 0;
+    for (uint32_t J = 0; J < nb; ++J) {
+        for (uint32_t I = 0; I <= J; ++I, ++tile) {
+            if (tile % nwaves != wave) continue;
+            const uint32_t i = 32 * I + (lane & 31), j = 32 * J + (lane & 31);
+            kz_v16i c;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) c[r] = 0;
+            for (uint32_t kb = J; kb < nb; ++kb) {
+                const uint32_t sh = 32 * (kb & 1) + 16 * (lane >> 5);
+                const size_t w = size_t(kb >> 1) * ws;
+                const unsigned long long wi = i < d ? rows[w + i] : 0ull;
+                const unsigned long long wj = j < d ? rows[w + j] : 0ull;
+                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi >> sh) & 0xFFFFu),
+                                                          bits_to_bytes(uint32_t(wj >> sh) & 0xFFFFu), c, 0, 0, 0);
+            }
+            // mask: A[i_r][j] for this lane's 16 rows i_r and its column j
+            if (j < d) {
+                const unsigned long long* col_word = rows + size_t(j >> 6) * ws;
+                const uint32_t bit = j & 63;
+                const uint32_t ib = 32 * I + (lane >> 5) * 4;
+                uint32_t s = 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
+                    const uint32_t a = ir < d ? uint32_t(col_word[ir] >> bit) & 1u : 0u;
+                    s += a * uint32_t(c[r]);
+                }
+                acc += s;
+            }
+        }
+    }
+}
+
 // Builds rows [r0, r1) of the DAG of root with list L (d ids, set rs) into
 // rows_out (word kk of row r at rows_out[kk * ws + r - r0], zeroed by the
 // caller), streaming the neighbour lists of all rows as 16-byte groups.  rb holds
@@ -339,7 +401,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void 
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
         __syncthreads();
         build_rows_block<K>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, acc);
-        if (K >= 4) count_rows<K>(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
+        if (K == 4) count_rows_mfma4(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
+        else if (K == 5) count_rows<K>(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
         __syncthreads();
     }
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
