@@ -104,6 +104,14 @@ int kz_csr_check(const kz_csr* g);
  * independent) used to prove that two builds produced the same graph. */
 uint64_t kz_csr_digest(const kz_csr* g);
 
+/* Attach vertex labels (labels[num_vertices], copied; replaces any held)
+ * to a host CSR: .dgraph files store them after the header
+ * (graph_loader.h:248-257) and kz_graph_upload uploads them.  Labels are
+ * LabelId (u32) values below KZ_MAX_LABELS (MAX_NUM_LABELS,
+ * graph_loader.h:28). */
+#define KZ_MAX_LABELS 48
+int kz_csr_set_labels(kz_csr* g, const uint32_t* labels);
+
 /* ------------------------------------------------------------------------ */
 /* Device runtime                                                            */
 /* ------------------------------------------------------------------------ */
@@ -243,6 +251,52 @@ int kz_clique_workload(kz_graph g, uint64_t* out6);
  * (KZ_ERR_INVALID). */
 #define KZ_CENSUS_PATTERNS 8
 int kz_motif_census(kz_graph g, uint64_t* counts, kz_run_stats* stats);
+
+/* ---------------------------------------------- labelled pattern mining -- */
+
+/* Vertex labels of a resident symmetric graph (labels[num_vertices], each
+ * below KZ_MAX_LABELS), uploaded with the label-sorted neighbour lists that
+ * frequent subgraph mining reads.  kz_graph_upload calls it for a kz_csr
+ * that carries labels.  Replaces the labelled load of
+ * DistributedApplication(..., load_vertex_labels = true)
+ * (distributed_application.h:38-46, graph_loader.h:248-257, :379-420). */
+int kz_graph_set_labels(kz_graph g, const uint32_t* labels);
+
+/* Levels of kz_fsm_mine, in the order kautomine/three_fsm.cc mines them. */
+#define KZ_FSM_VERTEX 0
+#define KZ_FSM_EDGE 1
+#define KZ_FSM_WEDGE 2       /* labels: center, leaf, leaf               */
+#define KZ_FSM_TRIANGLE 3
+#define KZ_FSM_FOUR_CHAIN 4  /* labels of the path v2 - v0 - v1 - v3: v0, v1, v2, v3 */
+#define KZ_FSM_THREE_STAR 5  /* labels: leaf, center, leaf, leaf        */
+#define KZ_FSM_LEVELS 6
+
+typedef struct kz_fsm_stats {
+    uint32_t num_frequent[KZ_FSM_LEVELS]; /* "Number of frequent ... patterns" */
+    uint64_t support_sum[KZ_FSM_LEVELS];  /* "support sum"; level 0 sums every label */
+    uint32_t num_candidates[KZ_FSM_LEVELS];
+    uint32_t num_labels;
+    double device_ms; /* domain kernels and popcounts            */
+    double total_ms;  /* including candidate generation         */
+} kz_fsm_stats;
+
+typedef struct kz_fsm_candidate {
+    uint32_t level;     /* KZ_FSM_EDGE .. KZ_FSM_THREE_STAR          */
+    uint32_t labels[4]; /* canonical labels (unused positions 0)     */
+    uint32_t support;   /* MNI: smallest per-position domain         */
+    uint32_t frequent;  /* support >= threshold                      */
+} kz_fsm_candidate;
+
+/* Frequent labelled 1..4-vertex patterns with minimum-image support >=
+ * threshold, level by level as kautomine/three_fsm.cc
+ * mining_frequent_patterns (:688-695) with its candidate generators
+ * (:276-604) and DistributedPatternDomain::get_support (engine.h:476-560).
+ * Every evaluated candidate is written to cands (up to cap, NULL allowed);
+ * *ncand (may be NULL) gets their number.  Needs kz_graph_set_labels.
+ * The reference prints the stats' num_frequent / support_sum per level;
+ * it has no other output. */
+int kz_fsm_mine(kz_graph g, uint32_t threshold, kz_fsm_stats* stats, kz_fsm_candidate* cands, uint32_t cap,
+                uint32_t* ncand);
 
 /* ---- Serving --------------------------------------------------------------
  * A server runs queries on resident graphs from a worker thread.  Queries

@@ -73,6 +73,21 @@ class CSRGraph:
     def degrees(self) -> np.ndarray:
         return np.diff(self.rowptr).astype(np.uint32)
 
+    @property
+    def labels(self):
+        """Vertex labels (u32 view) or None."""
+        if not self._raw.has_labels or not self._raw.labels:
+            return None
+        return np.ctypeslib.as_array(self._raw.labels, (max(1, self.num_vertices),))[: self.num_vertices]
+
+    def set_labels(self, labels) -> "CSRGraph":
+        """Attach vertex labels (copied; each below KZ_MAX_LABELS = 48)."""
+        lab = np.ascontiguousarray(labels, dtype=np.uint32)
+        if lab.shape != (self.num_vertices,):
+            raise ValueError(f"need {self.num_vertices} labels, got {lab.shape}")
+        check(lib().kz_csr_set_labels(C.byref(self._raw), lab.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return self
+
     def oriented(self) -> "CSRGraph":
         """Degree orientation (to_oriented_dataset_converter.cc:27-124)."""
         out = _ffi.kz_csr()
@@ -391,6 +406,32 @@ class DeviceGraph:
         check(lib().kz_motif_census(self._h, out, C.byref(st)))
         self.last_stats = st.as_dict()
         return [int(x) for x in out]
+
+    def set_labels(self, labels) -> "DeviceGraph":
+        """Vertex labels for frequent subgraph mining (kz_graph_set_labels;
+        symmetric graphs, each label below 48)."""
+        lab = np.ascontiguousarray(labels, dtype=np.uint32)
+        if lab.shape != (self.num_vertices,):
+            raise ValueError(f"need {self.num_vertices} labels, got {lab.shape}")
+        check(lib().kz_graph_set_labels(self._h, lab.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return self
+
+    def fsm(self, threshold: int, cap: int = 1 << 20) -> dict:
+        """Frequent labelled 1-4 vertex patterns with MNI support >= threshold
+        (kz_fsm_mine, kautomine/three_fsm.cc): per level of _ffi.FSM_LEVELS
+        the number of frequent patterns and their support sum (what the
+        reference app prints), plus every evaluated candidate as (level,
+        labels, support, frequent)."""
+        st = _ffi.kz_fsm_stats()
+        buf = (_ffi.kz_fsm_candidate * cap)()
+        nc = C.c_uint32(0)
+        check(lib().kz_fsm_mine(self._h, threshold, C.byref(st), buf, cap, C.byref(nc)))
+        k = min(int(nc.value), cap)
+        cands = [(int(c.level), tuple(int(x) for x in c.labels), int(c.support), bool(c.frequent)) for c in buf[:k]]
+        self.last_stats = {"device_ms": st.device_ms, "total_ms": st.total_ms, "num_labels": int(st.num_labels),
+                           "num_candidates": [int(x) for x in st.num_candidates]}
+        return {"counts": [int(x) for x in st.num_frequent], "sums": [int(x) for x in st.support_sum],
+                "candidates": cands, "num_candidates": int(nc.value)}
 
     def count_plan(self, plan: _ffi.kz_plan_desc) -> list[int]:
         n = max(1, int(plan.num_aggregators))

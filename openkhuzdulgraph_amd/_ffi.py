@@ -112,6 +112,26 @@ class kz_bucket_cfg(C.Structure):
         return {"bound": list(self.bound), "block": list(self.block), "serial": int(self.serial)}
 
 
+KZ_FSM_LEVELS = 6
+FSM_LEVELS = ("vertex", "edge", "wedge", "triangle", "four_chain", "three_star")
+
+
+class kz_fsm_stats(C.Structure):
+    _fields_ = [
+        ("num_frequent", C.c_uint32 * KZ_FSM_LEVELS),
+        ("support_sum", C.c_uint64 * KZ_FSM_LEVELS),
+        ("num_candidates", C.c_uint32 * KZ_FSM_LEVELS),
+        ("num_labels", C.c_uint32),
+        ("device_ms", C.c_double),
+        ("total_ms", C.c_double),
+    ]
+
+
+class kz_fsm_candidate(C.Structure):
+    _fields_ = [("level", C.c_uint32), ("labels", C.c_uint32 * 4), ("support", C.c_uint32),
+                ("frequent", C.c_uint32)]
+
+
 class kz_server_cfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("reserved", C.c_uint32)]
 
@@ -187,6 +207,7 @@ _SIGS = {
     "kz_orient_by_degree": (C.c_int, [C.POINTER(kz_csr), C.POINTER(kz_csr)]),
     "kz_csr_check": (C.c_int, [C.POINTER(kz_csr)]),
     "kz_csr_digest": (C.c_uint64, [C.POINTER(kz_csr)]),
+    "kz_csr_set_labels": (C.c_int, [C.POINTER(kz_csr), C.POINTER(C.c_uint32)]),
     "kz_init": (C.c_int, [C.c_int]),
     "kz_shutdown": (C.c_int, []),
     "kz_get_device_info": (C.c_int, [C.POINTER(kz_device_info)]),
@@ -210,6 +231,9 @@ _SIGS = {
     "kz_set_level_budget": (C.c_int, [C.c_uint64]),
     "kz_clique_workload": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "kz_motif_census": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(kz_run_stats)]),
+    "kz_graph_set_labels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "kz_fsm_mine": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(kz_fsm_stats), C.POINTER(kz_fsm_candidate),
+                              C.c_uint32, C.POINTER(C.c_uint32)]),
     "kz_server_start": (C.c_int, [C.POINTER(kz_server_cfg), C.POINTER(C.c_void_p)]),
     "kz_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_uint64)]),
     "kz_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(kz_query_result)]),

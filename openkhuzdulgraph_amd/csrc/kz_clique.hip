@@ -505,7 +505,6 @@ __global__ __launch_bounds__(256) void hub_count_kernel(const HubMeta* __restric
                                                         uint32_t* __restrict__ cursor,
                                                         unsigned long long* __restrict__ results,
                                                         const unsigned long long* __restrict__ slab) {
-    const uint32_t lane = threadIdx.x & 63;
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
     TaskChunk chunk(nrows, gridDim.x * (blockDim.x >> 6), 8);

@@ -67,6 +67,9 @@ struct DeviceGraph {
     // motif-census state: the graph relabelled by (degree, id) rank, its
     // forward part, counter slabs and the 4-clique job (kz_census.hip)
     std::shared_ptr<void> census;
+    // vertex labels and the label-sorted lists of frequent subgraph mining
+    // (kz_fsm.hip), set by kz_graph_set_labels
+    std::shared_ptr<void> fsm;
     bool bucket_set[6] = {};
     uint32_t bucket_hi[6][5] = {};
     uint32_t bucket_block[6][5] = {};
@@ -79,6 +82,7 @@ struct Scratch {
     size_t size = 0;
 };
 int scratch_reserve(Scratch& s, size_t bytes);
+int fsm_set_labels(DeviceGraph& d, const uint32_t* labels);
 
 }  // namespace kz
 

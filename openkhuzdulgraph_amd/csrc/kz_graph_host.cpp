@@ -385,6 +385,20 @@ int kz_csr_check(const kz_csr* g) {
     return KZ_OK;
 }
 
+int kz_csr_set_labels(kz_csr* g, const uint32_t* labels) {
+    if (!g || !g->rowptr || (!labels && g->num_vertices)) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    for (uint32_t v = 0; v < g->num_vertices; ++v)
+        if (labels[v] >= KZ_MAX_LABELS)
+            KZ_FAIL(KZ_ERR_INVALID, "label %u of vertex %u is not below %d", labels[v], v, KZ_MAX_LABELS);
+    uint32_t* copy = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * (g->num_vertices ? g->num_vertices : 1)));
+    if (!copy) KZ_FAIL(KZ_ERR_NOMEM, "labels");
+    if (g->num_vertices) std::memcpy(copy, labels, sizeof(uint32_t) * g->num_vertices);
+    std::free(g->labels);
+    g->labels = copy;
+    g->has_labels = 1;
+    return KZ_OK;
+}
+
 uint64_t kz_csr_digest(const kz_csr* g) {
     if (!g || !g->rowptr) return 0;
     // Word-wise mix over (n, entries, rowptr[], col[]); chunks are hashed in

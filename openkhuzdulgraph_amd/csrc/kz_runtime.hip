@@ -148,6 +148,12 @@ int kz_graph_upload(const kz_csr* h, kz_graph* out) {
         maxd = d.h_deg[v] > maxd ? d.h_deg[v] : maxd;
     }
     d.max_degree = maxd;
+    if (h->has_labels && h->labels && !d.oriented) {
+        if (int e = fsm_set_labels(d, h->labels)) {
+            kz_graph_release(g);
+            return e;
+        }
+    }
     *out = g;
     return KZ_OK;
 }

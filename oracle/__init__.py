@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
         L.kzo_clique_bytes.argtypes = [U64P, U32P, C.c_uint32, C.c_int, C.c_int, U32P, C.c_uint64, U64P]
         L.kzo_clique_bytes.restype = C.c_int
         L.kzo_motif_census.argtypes = [U64P, U32P, C.c_uint32, C.c_int, U64P, U64P]
+        L.kzo_fsm.argtypes = [U64P, U32P, C.c_uint32, U32P, C.c_uint32, U32P, U64P, U32P, C.c_uint32, U32P]
         L.kzo_motif_census.restype = C.c_int
         _lib = L
     return _lib
@@ -230,3 +231,27 @@ def run_plan(plan, rowptr, col, threads: int = 0) -> list[int]:
     L.kzo_run_plan(_C.cast(cbuf, _C.c_void_p), _p64(rp), _p32(cl), len(rp) - 1, threads, _p64(raw))
     n = int.from_bytes(buf[4:8], "little")
     return [int(x) for x in raw[:n]]
+
+
+FSM_LEVELS = ("vertex", "edge", "wedge", "triangle", "four_chain", "three_star")
+
+
+def fsm(rowptr, col, labels, threshold: int, cap: int = 1 << 20) -> dict:
+    """kautomine/three_fsm.cc restated: per level (FSM_LEVELS) the number of
+    frequent patterns and their support sum, plus every evaluated candidate
+    as (level, labels, support, frequent)."""
+    L = lib()
+    rp = np.ascontiguousarray(rowptr, dtype=np.uint64)
+    cl = _u32(col)
+    lab = _u32(labels)
+    n = len(rp) - 1
+    counts = np.zeros(6, np.uint32)
+    sums = np.zeros(6, np.uint64)
+    cand = np.zeros((cap, 7), np.uint32)
+    nc = np.zeros(1, np.uint32)
+    L.kzo_fsm(_p64(rp), _p32(cl), n, _p32(lab), threshold, _p32(counts), _p64(sums), _p32(cand), cap, _p32(nc))
+    k = min(int(nc[0]), cap)
+    cands = [(int(r[0]), tuple(int(x) for x in r[1:5]), int(r[5]), bool(r[6])) for r in cand[:k]]
+    return {"counts": [int(x) for x in counts], "sums": [int(x) for x in sums], "candidates": cands,
+            "num_candidates": int(nc[0])}
+

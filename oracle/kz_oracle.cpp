@@ -14,6 +14,9 @@
 //     reference does not change counts, so the restatement walks the same
 //     tree depth-first;
 //   * each app's generated apply() schedule, cited per function below;
+//   * the labelled FSM app kautomine/three_fsm.cc (mining_frequent_patterns,
+//     :688-695): its candidate generators (:276-604), apply() (:52-250) and
+//     DistributedPatternDomain's MNI support (include/engine.h:405-580);
 //   * Aggregator<EdgeId>: a uint64 sum wrapping mod 2^64
 //     (include/engine.h:615-673), divided by the app's multiplicity
 //     (kautomine/three_motif.cc:44, :280-284).
@@ -815,4 +818,215 @@ int kzo_motif_census(const uint64_t* rowptr, const uint32_t* col, uint32_t n, in
     return 8;
 }
 
+
+// ------------------------------------------------------------ three_fsm --
+// Frequent 3/4-vertex labelled patterns with MNI support, restating
+// kautomine/three_fsm.cc.  Levels: 0 vertex, 1 edge, 2 wedge, 3 triangle,
+// 4 four-chain, 5 three-star.  counts[l] / sums[l] are what the app prints
+// for level l ("Number of frequent ... patterns: X, support sum ...: Y";
+// level 0 sums the support of every label, :260-269).  Every evaluated
+// candidate is appended to cand (if not NULL, up to cap entries) as
+// {level, l0, l1, l2, l3, support, frequent}; *ncand gets the total.
+// Each candidate is enumerated literally as apply() does (roots are the
+// vertices of label labels[0], engine.cc:1459-1465), and a domain keeps the
+// distinct vertices inserted with enable = true at each pattern position
+// (engine.h:465-475); the support is the smallest domain (:476-560).
 }  // extern "C"
+
+#include <set>
+
+namespace {
+struct FsmPat {
+    int size;
+    uint32_t l[4];
+    bool operator<(const FsmPat& o) const {
+        if (size != o.size) return size < o.size;
+        for (int i = 0; i < size; ++i)
+            if (l[i] != o.l[i]) return l[i] < o.l[i];
+        return false;
+    }
+};
+struct FsmCand {
+    FsmPat canon;
+    std::vector<FsmPat> autos;
+};
+FsmPat fp(uint32_t a, uint32_t b) { return FsmPat{2, {a, b, 0, 0}}; }
+FsmPat fp(uint32_t a, uint32_t b, uint32_t c) { return FsmPat{3, {a, b, c, 0}}; }
+FsmPat fp(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return FsmPat{4, {a, b, c, d}}; }
+}  // namespace
+
+extern "C" {
+
+int kzo_fsm(const uint64_t* rowptr, const uint32_t* col, uint32_t n, const uint32_t* lab, uint32_t threshold,
+            uint32_t* counts, uint64_t* sums, uint32_t* cand, uint32_t cap, uint32_t* ncand) {
+    uint32_t L = 0;
+    for (uint32_t v = 0; v < n; ++v) L = std::max(L, lab[v] + 1);
+    std::vector<uint32_t> lc(L, 0);
+    for (uint32_t v = 0; v < n; ++v) ++lc[lab[v]];
+    std::vector<char> fv(L);
+    counts[0] = 0;
+    sums[0] = 0;
+    for (uint32_t l = 0; l < L; ++l) {  // setup(), :255-275
+        fv[l] = lc[l] >= threshold;
+        counts[0] += fv[l];
+        sums[0] += lc[l];
+    }
+    std::set<FsmPat> fe, fw;  // frequent edge / wedge tables (with automorphisms)
+    uint32_t nc = 0;
+    auto nbr = [&](uint32_t v) { return std::make_pair(col + rowptr[v], col + rowptr[v + 1]); };
+    std::vector<std::vector<char>> dom(4, std::vector<char>(n));
+    std::vector<uint32_t> dsz(4);
+    auto ins = [&](int i, uint32_t v, bool en) {  // DistributedPatternDomain::insert
+        if (en && !dom[i][v]) {
+            dom[i][v] = 1;
+            ++dsz[i];
+        }
+    };
+    // one candidate: enumerate as apply() does, return the MNI support
+    auto support = [&](int level, const FsmPat& p) -> uint32_t {
+        for (int i = 0; i < 4; ++i) std::fill(dom[i].begin(), dom[i].end(), 0), dsz[i] = 0;
+        for (uint32_t v0 = 0; v0 < n; ++v0) {
+            if (lab[v0] != p.l[0]) continue;
+            auto [b0, e0] = nbr(v0);
+            if (level == 1) {  // Edge, :55-68
+                for (auto a = b0; a != e0; ++a) {
+                    const bool en = lab[*a] == p.l[1];
+                    ins(0, v0, en), ins(1, *a, en);
+                }
+                continue;
+            }
+            for (auto a = b0; a != e0; ++a) {
+                const uint32_t v1 = *a;
+                if (lab[v1] != p.l[1]) continue;
+                if (level == 2) {  // Wedge, :70-103
+                    for (auto c = b0; c != e0; ++c) {
+                        const bool en = lab[*c] == p.l[2] && *c != v1;
+                        ins(0, v0, en), ins(1, v1, en), ins(2, *c, en);
+                    }
+                } else if (level == 3) {  // Triangle, :105-147
+                    auto [b1, e1] = nbr(v1);
+                    std::vector<uint32_t> t;
+                    std::set_intersection(b0, e0, b1, e1, std::back_inserter(t));
+                    for (uint32_t v2 : t) {
+                        const bool en = lab[v2] == p.l[2];
+                        ins(0, v0, en), ins(1, v1, en), ins(2, v2, en);
+                    }
+                } else if (level == 4) {  // FourChain, :149-197
+                    auto [b1, e1] = nbr(v1);
+                    for (auto c = b0; c != e0; ++c) {
+                        const uint32_t v2 = *c;
+                        if (lab[v2] != p.l[2] || v1 == v2) continue;
+                        for (auto d = b1; d != e1; ++d) {
+                            const uint32_t v3 = *d;
+                            const bool en = lab[v3] == p.l[3] && v3 != v0 && v3 != v2;
+                            ins(0, v0, en), ins(1, v1, en), ins(2, v2, en), ins(3, v3, en);
+                        }
+                    }
+                } else {  // ThreeStar, :199-247
+                    auto [b1, e1] = nbr(v1);
+                    for (auto c = b1; c != e1; ++c) {
+                        const uint32_t v2 = *c;
+                        if (lab[v2] != p.l[2] || v0 == v2) continue;
+                        for (auto d = b1; d != e1; ++d) {
+                            const uint32_t v3 = *d;
+                            const bool en = lab[v3] == p.l[3] && v3 != v0 && v3 != v2;
+                            ins(0, v0, en), ins(1, v1, en), ins(2, v2, en), ins(3, v3, en);
+                        }
+                    }
+                }
+            }
+        }
+        uint32_t s = n;  // get_support, :476-560
+        for (int i = 0; i < p.size; ++i) s = std::min(s, dsz[i]);
+        return s;
+    };
+    auto evaluate = [&](int level, const std::vector<FsmCand>& cs, std::set<FsmPat>* table) {
+        counts[level] = 0;
+        sums[level] = 0;
+        for (const FsmCand& c : cs) {
+            const uint32_t s = support(level, c.canon);
+            const bool fr = s >= threshold;
+            if (cand && nc < cap) {
+                uint32_t* o = cand + size_t(nc) * 7;
+                o[0] = uint32_t(level);
+                for (int i = 0; i < 4; ++i) o[1 + i] = i < c.canon.size ? c.canon.l[i] : 0u;
+                o[5] = s;
+                o[6] = fr;
+            }
+            ++nc;
+            if (fr) {
+                if (table) {
+                    table->insert(c.canon);
+                    for (const FsmPat& a : c.autos) table->insert(a);
+                }
+                ++counts[level];
+                sums[level] += s;
+            }
+        }
+    };
+    auto has = [](const std::set<FsmPat>& t, const FsmPat& p) { return t.count(p) != 0; };
+    std::vector<FsmCand> cs;
+    // edges, :282-304
+    for (uint32_t a = 0; a < L; ++a)
+        for (uint32_t b = 0; b < L; ++b)
+            if (fv[a] && fv[b] && a <= b) cs.push_back({fp(a, b), {fp(b, a)}});
+    evaluate(1, cs, &fe);
+    // wedges, :306-339 (center first)
+    cs.clear();
+    for (uint32_t a = 0; a < L; ++a) {
+        if (!fv[a]) continue;
+        for (uint32_t b = 0; b < L; ++b) {
+            if (!has(fe, fp(a, b))) continue;
+            for (uint32_t c = 0; c < L; ++c)
+                if (has(fe, fp(a, c)) && b <= c) cs.push_back({fp(a, b, c), {fp(a, c, b)}});
+        }
+    }
+    evaluate(2, cs, &fw);
+    // triangles, :341-399 (the table of frequent triangles is not read later)
+    cs.clear();
+    for (uint32_t a = 0; a < L; ++a) {
+        if (!fv[a]) continue;
+        for (uint32_t b = 0; b < L; ++b) {
+            if (!has(fe, fp(a, b))) continue;
+            for (uint32_t c = 0; c < L; ++c)
+                if (has(fw, fp(a, b, c)) && has(fw, fp(b, a, c)) && has(fw, fp(c, a, b)) && a <= b && b <= c)
+                    cs.push_back({fp(a, b, c), {}});
+        }
+    }
+    evaluate(3, cs, nullptr);
+    // four-chains v2 - v0 - v1 - v3, :401-459
+    cs.clear();
+    for (uint32_t a = 0; a < L; ++a) {
+        if (!fv[a]) continue;
+        for (uint32_t b = 0; b < L; ++b) {
+            if (!has(fe, fp(a, b))) continue;
+            for (uint32_t c = 0; c < L; ++c) {
+                if (!has(fw, fp(a, b, c))) continue;
+                for (uint32_t d = 0; d < L; ++d)
+                    if (has(fw, fp(b, a, d)) && has(fe, fp(a, c)) && has(fe, fp(b, d)) && a <= b)
+                        cs.push_back({fp(a, b, c, d), {}});
+            }
+        }
+    }
+    evaluate(4, cs, nullptr);
+    // three-stars (leaf, center, leaf, leaf), :461-533
+    cs.clear();
+    for (uint32_t a = 0; a < L; ++a) {
+        if (!fv[a]) continue;
+        for (uint32_t b = 0; b < L; ++b) {
+            if (!has(fe, fp(a, b))) continue;
+            for (uint32_t c = 0; c < L; ++c) {
+                if (!has(fw, fp(a, b, c))) continue;
+                for (uint32_t d = 0; d < L; ++d)
+                    if (has(fw, fp(a, c, d)) && has(fw, fp(a, b, d)) && b <= c && c <= d)
+                        cs.push_back({fp(b, a, c, d), {}});
+            }
+        }
+    }
+    evaluate(5, cs, nullptr);
+    if (ncand) *ncand = nc;
+    return 0;
+}
+
+}  // extern "C"
+

@@ -216,3 +216,60 @@ def test_census_non_induced_identities():
          [0, 0, 0, 0, 0, 1]]
     x = [p3, st, c4, paw, dia, k4]
     assert [sum(A[i][j] * x[j] for j in range(6)) for i in range(6)] == non[2:]
+
+
+# --------------------------------------------------------------- three_fsm --
+FSM_GOLD = json.loads((Path(__file__).parent / "golden" / "golden_fsm.json").read_text())
+FSM_TEMPLATES = {  # level -> (pattern size, edges between positions)
+    1: (2, [(0, 1)]),
+    2: (3, [(0, 1), (0, 2)]),
+    3: (3, [(0, 1), (0, 2), (1, 2)]),
+    4: (4, [(0, 1), (0, 2), (1, 3)]),
+    5: (4, [(1, 0), (1, 2), (1, 3)]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FSM_GOLD))
+def test_oracle_fsm_matches_reference_golden(name):
+    """Frequent-pattern counts and support sums per level equal what the
+    reference kautomine three_fsm binary printed for the same labelled
+    .dgraph and threshold."""
+    e = FSM_GOLD[name]
+    g = kz.load_dgraph(str(Path(__file__).parent / "golden" / e["file"]))
+    assert g.labels is not None and int(g.labels.max()) + 1 == e["num_labels"]
+    for t, want in e["three_fsm"].items():
+        got = oracle.fsm(g.rowptr, g.col, g.labels, int(t))
+        assert got["counts"] == want["counts"], (name, t)
+        assert got["sums"] == want["sums"], (name, t)
+
+
+def _brute_support(n, adj, lab, level, labels):
+    """MNI support of one labelled template by brute force: every injective
+    assignment of graph vertices to pattern positions with the template's
+    edges present and the labels matching."""
+    from itertools import permutations
+    size, edges = FSM_TEMPLATES[level]
+    dom = [set() for _ in range(size)]
+    for t in permutations(range(n), size):
+        if all(lab[t[i]] == labels[i] for i in range(size)) and all(t[b] in adj[t[a]] for a, b in edges):
+            for i in range(size):
+                dom[i].add(t[i])
+    return min([n] + [len(d) for d in dom])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_oracle_fsm_brute_force(seed):
+    """Every candidate's support equals a brute-force minimum-image count
+    over injective embeddings on small random labelled graphs (thresholds
+    low enough that every level has candidates)."""
+    rng = np.random.default_rng(seed)
+    g = kz.gen_erdos_renyi(11, 30, seed)
+    n = g.num_vertices
+    lab = rng.integers(0, 2 + seed % 2, n).astype(np.uint32)
+    rp, col = g.rowptr, g.col
+    adj = [set(int(x) for x in col[rp[v]:rp[v + 1]]) for v in range(n)]
+    r = oracle.fsm(rp, col, lab, 1)
+    assert r["num_candidates"] > 10
+    for level, labels, sup, fr in r["candidates"]:
+        assert sup == _brute_support(n, adj, lab, level, labels), (level, labels)
+        assert fr == (sup >= 1)
