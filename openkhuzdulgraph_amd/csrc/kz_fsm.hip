@@ -16,7 +16,8 @@
 //     is decided by existence rules over label-filtered neighbourhoods: the
 //     neighbour lists are stored label-sorted (ids ascending within a label),
 //     so N_l(v), the neighbours of v with label l, is one contiguous segment
-//     found by two binary searches over the entries' labels;
+//     whose bounds are two words of a per-vertex table of label offsets
+//     (L + 1 u32 per vertex, written by the same sort);
 //   * one wave per root vertex (the vertices of the root label); domains are
 //     bitmaps in HBM (one per candidate and position, set with atomicOr) and
 //     their sizes come from one popcount pass per level;
@@ -61,14 +62,14 @@ struct FsmState {
     uint32_t n = 0, L = 0;
     uint32_t* lab = nullptr;    // labels[n]
     uint32_t* lcol = nullptr;   // neighbour lists sorted by (label, id), same rowptr
-    uint8_t* llab = nullptr;    // label of each lcol entry
+    uint32_t* loff = nullptr;   // per vertex, L + 1 offsets of the label segments in its row
     uint32_t* vlist = nullptr;  // vertices grouped by label
     std::vector<uint32_t> voff;    // host: vlist offsets per label (L + 1)
     uint64_t bytes = 0;
     ~FsmState() {
         hipFree(lab);
         hipFree(lcol);
-        hipFree(llab);
+        hipFree(loff);
         hipFree(vlist);
     }
 };
@@ -77,11 +78,12 @@ namespace {
 
 // One wave per row: stable counting sort of the row's entries by label
 // (labels < 64), through a per-wave LDS histogram and, chunk by chunk, one
-// ballot per distinct label in the chunk.
+// ballot per distinct label in the chunk; the histogram's exclusive prefix is
+// the row's label offsets (loff[v * (L + 1) + l], l = 0..L).
 __global__ __launch_bounds__(256) void label_sort_kernel(const uint64_t* __restrict__ rowptr,
                                                          const uint32_t* __restrict__ col,
-                                                         const uint32_t* __restrict__ lab, uint32_t n,
-                                                         uint32_t* __restrict__ lcol, uint8_t* __restrict__ llab) {
+                                                         const uint32_t* __restrict__ lab, uint32_t n, uint32_t L,
+                                                         uint32_t* __restrict__ lcol, uint32_t* __restrict__ loff) {
     __shared__ uint32_t sBase[4][64];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t* base = sBase[w];
@@ -104,6 +106,7 @@ __global__ __launch_bounds__(256) void label_sort_kernel(const uint64_t* __restr
         }
         wave_lds_sync();
         base[lane] = excl;
+        if (lane <= L) loff[size_t(v) * (L + 1) + lane] = excl;
         wave_lds_sync();
         for (uint64_t k0 = b; k0 < e; k0 += 64) {
             const uint64_t k = k0 + lane;
@@ -119,7 +122,6 @@ __global__ __launch_bounds__(256) void label_sort_kernel(const uint64_t* __restr
                 if (l == ll) {
                     const uint64_t pos = b + at + uint32_t(__popcll(m & lt));
                     lcol[pos] = x;
-                    llab[pos] = uint8_t(ll);
                 }
                 wave_lds_sync();
                 if (lane == lead) base[ll] = at + uint32_t(__popcll(m));
@@ -136,23 +138,11 @@ struct Seg {
 };
 
 // N_l(v): the entries of label l in v's label-sorted list.
-__device__ __forceinline__ Seg seg_of(const uint64_t* __restrict__ rowptr, const uint8_t* __restrict__ llab,
-                                      uint32_t v, uint32_t l) {
-    uint64_t lo = ldg(rowptr, v), hi = ldg(rowptr, v + 1);
-    uint64_t a = lo, z = hi;
-    while (a < z) {
-        const uint64_t m = (a + z) >> 1;
-        if (ldg(llab, m) < l) a = m + 1;
-        else z = m;
-    }
-    const uint64_t b = a;
-    z = hi;
-    while (a < z) {
-        const uint64_t m = (a + z) >> 1;
-        if (ldg(llab, m) <= l) a = m + 1;
-        else z = m;
-    }
-    return Seg{b, a};
+__device__ __forceinline__ Seg seg_of(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ loff,
+                                      uint32_t L1, uint32_t v, uint32_t l) {
+    const uint64_t b = ldg(rowptr, v);
+    const uint32_t* o = loff + size_t(v) * L1 + l;
+    return Seg{b + ldg(o, 0), b + ldg(o, 1)};
 }
 
 __device__ __forceinline__ void mark(uint32_t* bm, uint32_t v) { atomicOr(&bm[v >> 5], 1u << (v & 31)); }
@@ -187,7 +177,8 @@ __device__ __forceinline__ bool seg_has(const uint32_t* __restrict__ lcol, Seg s
 struct FsmArgs {
     const uint64_t* rowptr;
     const uint32_t* lcol;
-    const uint8_t* llab;
+    const uint32_t* loff;
+    uint32_t L1;  // labels + 1: the offset table's row length
     const uint32_t* roots;
     uint32_t nroots;
     uint32_t l[4];
@@ -224,20 +215,20 @@ __global__ __launch_bounds__(256) void fsm_domain_kernel(FsmArgs a) {
     for (uint32_t r = wid; r < a.nroots; r += nw) {
         const uint32_t v = ldg(a.roots, r);
         if (T == 1) {  // edge: v = v0
-            const Seg s1 = seg_of(a.rowptr, a.llab, v, l1);
+            const Seg s1 = seg_of(a.rowptr, a.loff, a.L1, v, l1);
             if (s1.e > s1.b) {
                 if (lane == 0) mark(D0, v);
                 mark_seg(D1, a.lcol, s1, kNone, kNone);
             }
         } else if (T == 2) {  // wedge: v = v0 (center)
-            const Seg s1 = seg_of(a.rowptr, a.llab, v, l1), s2 = seg_of(a.rowptr, a.llab, v, l2);
+            const Seg s1 = seg_of(a.rowptr, a.loff, a.L1, v, l1), s2 = seg_of(a.rowptr, a.loff, a.L1, v, l2);
             if (s1.e > s1.b && s2.e - s2.b >= 1 + uint64_t(l1 == l2)) {
                 if (lane == 0) mark(D0, v);
                 mark_seg(D1, a.lcol, s1, kNone, kNone);
                 mark_seg(D2, a.lcol, s2, kNone, kNone);
             }
         } else if (T == 3) {  // triangle: v = v0, lanes take v1
-            const Seg A = seg_of(a.rowptr, a.llab, v, l2), S1 = seg_of(a.rowptr, a.llab, v, l1);
+            const Seg A = seg_of(a.rowptr, a.loff, a.L1, v, l2), S1 = seg_of(a.rowptr, a.loff, a.L1, v, l1);
             bool any = false;
             if (A.e > A.b) {
                 for (uint64_t k0 = S1.b; k0 < S1.e; k0 += 64) {
@@ -245,7 +236,7 @@ __global__ __launch_bounds__(256) void fsm_domain_kernel(FsmArgs a) {
                     bool hit = false;
                     if (k < S1.e) {
                         const uint32_t v1 = ldg(a.lcol, k);
-                        const Seg B = seg_of(a.rowptr, a.llab, v1, l2);
+                        const Seg B = seg_of(a.rowptr, a.loff, a.L1, v1, l2);
                         // walk the shorter list, search the longer
                         const bool ab = A.e - A.b <= B.e - B.b;
                         const Seg sm = ab ? A : B, lg = ab ? B : A;
@@ -263,7 +254,7 @@ __global__ __launch_bounds__(256) void fsm_domain_kernel(FsmArgs a) {
             }
             if (any && lane == 0) mark(D0, v);
         } else if (T == 4) {  // four-chain v2 - v0 - v1 - v3, from the v0 side: v = v0
-            const Seg A2 = seg_of(a.rowptr, a.llab, v, l2), S1 = seg_of(a.rowptr, a.llab, v, l1);
+            const Seg A2 = seg_of(a.rowptr, a.loff, a.L1, v, l2), S1 = seg_of(a.rowptr, a.loff, a.L1, v, l1);
             const uint64_t s2 = (A2.e - A2.b) - uint64_t(l1 == l2);
             bool any = false;
             uint32_t i1 = kNone, i2 = kNone;
@@ -274,7 +265,7 @@ __global__ __launch_bounds__(256) void fsm_domain_kernel(FsmArgs a) {
                     uint32_t v1 = kNone, x2 = kNone;
                     if (k < S1.e) {
                         v1 = ldg(a.lcol, k);
-                        const Seg B3 = seg_of(a.rowptr, a.llab, v1, l3);
+                        const Seg B3 = seg_of(a.rowptr, a.loff, a.L1, v1, l3);
                         const uint64_t s3 = (B3.e - B3.b) - uint64_t(l0 == l3);
                         if (s3 >= 1) {
                             const uint32_t e3 = s3 == 1 ? only_other(a.lcol, B3, v) : kNone;
@@ -292,7 +283,7 @@ __global__ __launch_bounds__(256) void fsm_domain_kernel(FsmArgs a) {
                 mark_seg(D2, a.lcol, A2, i1, i2);
             }
         } else if (T == 5) {  // four-chain, from the v1 side: v = v1, lanes take v0
-            const Seg B3 = seg_of(a.rowptr, a.llab, v, l3), S0 = seg_of(a.rowptr, a.llab, v, l0);
+            const Seg B3 = seg_of(a.rowptr, a.loff, a.L1, v, l3), S0 = seg_of(a.rowptr, a.loff, a.L1, v, l0);
             const uint64_t s3 = (B3.e - B3.b) - uint64_t(l0 == l3);
             bool any = false;
             uint32_t j1 = kNone, j2 = kNone;
@@ -303,7 +294,7 @@ __global__ __launch_bounds__(256) void fsm_domain_kernel(FsmArgs a) {
                     uint32_t v0 = kNone, y2 = kNone;
                     if (k < S0.e) {
                         v0 = ldg(a.lcol, k);
-                        const Seg A2 = seg_of(a.rowptr, a.llab, v0, l2);
+                        const Seg A2 = seg_of(a.rowptr, a.loff, a.L1, v0, l2);
                         const uint64_t s2 = (A2.e - A2.b) - uint64_t(l1 == l2);
                         if (s2 >= 1) {
                             const uint32_t e2 = s2 == 1 ? only_other(a.lcol, A2, v) : kNone;
@@ -317,8 +308,8 @@ __global__ __launch_bounds__(256) void fsm_domain_kernel(FsmArgs a) {
             }
             if (any) mark_seg(D3, a.lcol, B3, j1, j2);
         } else {  // three-star: v = v1 (center), leaves of labels l0, l2, l3
-            const Seg S0 = seg_of(a.rowptr, a.llab, v, l0), S2 = seg_of(a.rowptr, a.llab, v, l2),
-                      S3 = seg_of(a.rowptr, a.llab, v, l3);
+            const Seg S0 = seg_of(a.rowptr, a.loff, a.L1, v, l0), S2 = seg_of(a.rowptr, a.loff, a.L1, v, l2),
+                      S3 = seg_of(a.rowptr, a.loff, a.L1, v, l3);
             const uint64_t r0 = 1 + uint64_t(l2 == l0) + uint64_t(l3 == l0);
             const uint64_t r2 = 1 + uint64_t(l0 == l2) + uint64_t(l3 == l2);
             const uint64_t r3 = 1 + uint64_t(l0 == l3) + uint64_t(l2 == l3);
@@ -382,19 +373,20 @@ int fsm_set_labels(DeviceGraph& d, const uint32_t* h_lab) {
     }
     const size_t n1 = std::max<uint32_t>(1, d.n);
     const size_t m1 = std::max<uint64_t>(1, d.nnz);
+    const size_t offs = size_t(F->L + 1) * n1;
     if (hipMalloc(&F->lab, 4 * n1) != hipSuccess || hipMalloc(&F->vlist, 4 * n1) != hipSuccess ||
-        hipMalloc(&F->lcol, 4 * m1) != hipSuccess || hipMalloc(&F->llab, m1) != hipSuccess) {
+        hipMalloc(&F->lcol, 4 * m1) != hipSuccess || hipMalloc(&F->loff, 4 * offs) != hipSuccess) {
         (void)hipGetLastError();
         KZ_FAIL(KZ_ERR_NOMEM, "labelled lists of %llu entries", (unsigned long long)d.nnz);
     }
-    F->bytes = 8 * n1 + 5 * m1;
+    F->bytes = 8 * n1 + 4 * m1 + 4 * offs;
     hipStream_t s = rt().stream;
     if (d.n) {
         KZ_HIP(hipMemcpyAsync(F->lab, h_lab, 4 * size_t(d.n), hipMemcpyHostToDevice, s));
         KZ_HIP(hipMemcpyAsync(F->vlist, vl.data(), 4 * size_t(d.n), hipMemcpyHostToDevice, s));
         const uint32_t grid = std::min<uint32_t>((d.n + 3) / 4, uint32_t(rt().num_cus) * 16);
-        hipLaunchKernelGGL(label_sort_kernel, dim3(grid), dim3(256), 0, s, d.rowptr, d.col, F->lab, d.n, F->lcol,
-                           F->llab);
+        hipLaunchKernelGGL(label_sort_kernel, dim3(grid), dim3(256), 0, s, d.rowptr, d.col, F->lab, d.n, F->L,
+                           F->lcol, F->loff);
         KZ_HIP(hipGetLastError());
     }
     KZ_HIP(hipStreamSynchronize(s));
@@ -462,7 +454,7 @@ int fsm_mine(DeviceGraph& d, uint32_t threshold, kz_fsm_stats* st, kz_fsm_candid
             KZ_HIP(hipMemsetAsync(sz, 0, 16 * size_t(nb), s));
             for (uint32_t c = 0; c < nb; ++c) {
                 const Pat& p = cs[c0 + c].canon;
-                FsmArgs a{d.rowptr, F.lcol, F.llab, nullptr, 0, {0, 0, 0, 0}, bm + size_t(c) * 4 * W, W};
+                FsmArgs a{d.rowptr, F.lcol, F.loff, L + 1, nullptr, 0, {0, 0, 0, 0}, bm + size_t(c) * 4 * W, W};
                 for (int i = 0; i < p.size; ++i) a.l[i] = p.l[i];
                 // roots: the vertices of the first label; the three-star and
                 // the four-chain's second pass start from their center / v1
