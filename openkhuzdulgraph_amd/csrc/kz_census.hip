@@ -155,7 +155,7 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);
     p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
-    __shared__ uint32_t sTask, sTmp[T / 64];
+    __shared__ uint32_t sTask, sTmp[T / 64], sSpan;
     const RootSet rs{hk, hv, H - 1, hshift, fb, fshift};
     TaskChunk chunk(ntasks, gridDim.x, T == 64 ? 2 : 8);
     while (true) {
@@ -174,6 +174,7 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < d; i += T) root_insert(rs, L[i], uint16_t(i));
         const uint32_t nr = d - 1;
+        if (threadIdx.x == 0) sSpan = 0;
         for (uint32_t r = threadIdx.x; r <= nr; r += T) {
             uint32_t groups = 0, len = 0;
             uint64_t desc = 0;
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
             atomicAdd(&cnt[j], 1u);
             atomicAdd(&tmid[at], 1u);
         };
-        stream_groups(fcol, rb, rp, nr, G, threadIdx.x, T, rs, L[d - 1], hit);
+        stream_groups(fcol, rb, rp, nr, G, threadIdx.x, T, rs, L[d - 1], hit, T > 64 ? &sSpan : nullptr);
         // skewed rows: each wave searches the candidates of its own rows
         search_skewed(fcol, rb, L, d, 0, nr, threadIdx.x >> 6, T / 64, hit);
         __syncthreads();

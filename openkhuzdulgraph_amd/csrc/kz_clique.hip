@@ -273,16 +273,25 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
         for (uint32_t I = 0; I <= J; ++I, ++tile) {
             if (tile % nwaves != wave) continue;
             const uint32_t i = 32 * I + (lane & 31), j = 32 * J + (lane & 31);
+            // rows past d read row d - 1: their products land in tile rows
+            // and columns that the mask below leaves out
+            const uint32_t ic = i < d ? i : d - 1, jc = j < d ? j : d - 1;
+            const uint32_t hs = 16 * (lane >> 5);
+            // one 64-bit word = two 32-column blocks, read once
             kz_v16i c;
 #pragma unroll
             for (int r = 0; r < 16; ++r) c[r] = 0;
-            for (uint32_t kb = J; kb < nb; ++kb) {
-                const uint32_t sh = 32 * (kb & 1) + 16 * (lane >> 5);
-                const size_t w = size_t(kb >> 1) * ws;
-                const unsigned long long wi = i < d ? rows[w + i] : 0ull;
-                const unsigned long long wj = j < d ? rows[w + j] : 0ull;
-                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi >> sh) & 0xFFFFu),
-                                                          bits_to_bytes(uint32_t(wj >> sh) & 0xFFFFu), c, 0, 0, 0);
+            const uint32_t wend = (nb + 1) >> 1;
+            for (uint32_t kw = J >> 1; kw < wend; ++kw) {
+                const unsigned long long wi = rows[size_t(kw) * ws + ic];
+                const unsigned long long wj = rows[size_t(kw) * ws + jc];
+                if (2 * kw >= J)
+                    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi >> hs) & 0xFFFFu),
+                                                              bits_to_bytes(uint32_t(wj >> hs) & 0xFFFFu), c, 0, 0, 0);
+                if (2 * kw + 1 < nb)
+                    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi >> (32 + hs)) & 0xFFFFu),
+                                                              bits_to_bytes(uint32_t(wj >> (32 + hs)) & 0xFFFFu), c, 0,
+                                                              0, 0);
             }
             // mask: A[i_r][j] for this lane's 16 rows i_r and its column j
             if (j < d) {
@@ -306,13 +315,16 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
 // rows_out (word kk of row r at rows_out[kk * ws + r - r0], zeroed by the
 // caller), streaming the neighbour lists of all rows as 16-byte groups.  rb holds
 // (r1 - r0) row descriptors and rp (r1 - r0 + 1) group prefixes, both LDS
-// scratch; tmp holds one u32 per wave.  Block-wide.
+// scratch; tmp holds one u32 per wave and ctr one u32 (the stream's span
+// counter).  Block-wide.
 template <int K>
 __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
                                                  const uint32_t* L, uint32_t d, const RootSet& rs, uint32_t r0, uint32_t r1,
                                                  unsigned long long* rows_out, uint32_t ws, unsigned long long* rb,
-                                                 uint32_t* rp, uint32_t* tmp, unsigned long long& acc) {
+                                                 uint32_t* rp, uint32_t* tmp, uint32_t* ctr,
+                                                 unsigned long long& acc) {
     const uint32_t nr = r1 - r0;
+    if (threadIdx.x == 0) *ctr = 0;
     for (uint32_t r = threadIdx.x; r <= nr; r += blockDim.x) {
         const uint32_t i = r0 + r;
         uint32_t groups = 0, len = 0;
@@ -323,11 +335,13 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
     }
     __syncthreads();
     const uint32_t G = block_exclusive_scan(rp, nr + 1, tmp);
-    stream_groups<kGroups, (K >= 4)>(col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
-                  [&](uint32_t r, uint32_t j, uint64_t) {
-                      if (K == 3) acc += 1;
-                      else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
-                  });
+    stream_groups<kGroups, (K >= 4)>(
+        col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
+        [&](uint32_t r, uint32_t j, uint64_t) {
+            if (K == 3) acc += 1;
+            else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
+        },
+        ctr);
     // skewed rows: each wave searches the candidates of its own rows
     search_skewed(col, rb, L, d, r0, nr, threadIdx.x >> 6, blockDim.x >> 6, [&](uint32_t r, uint32_t j, uint64_t) {
         if (K == 3) acc += 1;
@@ -369,7 +383,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void 
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);
     p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
-    __shared__ uint32_t sTask, sTmp[16];
+    __shared__ uint32_t sTask, sTmp[16], sSpan;
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
     const RootSet rs{hk, hv, H - 1, hshift, fb, fshift};
@@ -400,7 +414,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void 
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
         __syncthreads();
-        build_rows_block<K>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, acc);
+        build_rows_block<K>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc);
         if (K == 4) count_rows_mfma4(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
         else if (K == 5) count_rows<K>(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
         __syncthreads();
@@ -440,7 +454,7 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);
     p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
-    __shared__ uint32_t sItem, sTmp[16];
+    __shared__ uint32_t sItem, sTmp[16], sSpan;
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
     uint32_t hcur = 0xFFFFFFFFu;
@@ -483,7 +497,7 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
         if (K >= 4)
             for (uint32_t i = threadIdx.x; i < (r1 - r0) * hm.W; i += blockDim.x) rbuf[i] = 0;
         __syncthreads();
-        build_rows_block<K>(rowptr, col, L, d, rs, r0, r1, rbuf, r1 - r0, rb, rp, sTmp, acc);
+        build_rows_block<K>(rowptr, col, L, d, rs, r0, r1, rbuf, r1 - r0, rb, rp, sTmp, &sSpan, acc);
         if (K >= 4) {
             // the slab is word-major over all d rows of the hub
             const uint32_t nr = r1 - r0;

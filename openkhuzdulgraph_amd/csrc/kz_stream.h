@@ -180,21 +180,134 @@ __device__ __forceinline__ uint32_t wave_search(const uint32_t* p, uint32_t lo, 
 // is a hit: hit(r, j, p) with p the entry's position in col (j is only
 // looked up when kIndex).
 //
-// Each wave takes a contiguous share of the groups and walks it NG * 64
-// groups per iteration, lane l taking groups l, l + 64, ...  The iteration's
-// first row is the previous iteration's last, its last row comes from one
-// wave_search, and each lane finds the rows of its NG groups with NG binary
-// searches run in lockstep.  After the NG 16-byte loads, the 4 * NG entries
-// go through the root set in batched phases, each issuing all its LDS reads
-// before it waits: filter words, then the first two hash slots of the
-// entries that pass the filter, then the indices of the keys found.  Only
-// keys displaced further (rare at a load factor of at most 1/2) probe on.
+// A wave walks NG * 64 groups per iteration (a span), lane l taking groups
+// l, l + 64, ...: the span's last row comes from one wave_search, and each
+// lane finds the rows of its NG groups with NG binary searches run in
+// lockstep.  After the NG 16-byte loads, the 4 * NG entries go through the
+// root set in batched phases, each issuing all its LDS reads before it
+// waits: filter words, then the first two hash slots of the entries that
+// pass the filter, then the indices of the keys found.  Only keys displaced
+// further (rare at a load factor of at most 1/2) probe on.
+//
+// Spans go to waves either as one contiguous share per wave (ctr == NULL;
+// a span's first row is the previous span's last), or, for workgroups,
+// from a counter in LDS (ctr, zeroed by the caller before a barrier), one
+// span at a time with the next one's atomic issued ahead: the waves then
+// finish together instead of waiting at the caller's barrier for the one
+// whose share probed most.
+template <int NG, bool kIndex, class Hit>
+__device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned long long* desc, const uint32_t* gp,
+                                            uint32_t g0, uint32_t g1, uint32_t rlo, uint32_t rhi, const RootSet& rs,
+                                            uint32_t lmax, Hit&& hit) {
+    constexpr int NE = 4 * NG;
+    const uint32_t lane = threadIdx.x & 63;
+    // rows of this lane's groups (past g1: the group g1, masked below)
+    uint32_t gg[NG], rr[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        const uint32_t g = g0 + uint32_t(k) * 64 + lane;
+        gg[k] = g < g1 ? g : g1;
+        rr[k] = rlo;
+    }
+    for (uint32_t len = rhi - rlo; len > 1;) {
+        const uint32_t half = len >> 1;
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+            const uint32_t m = rr[k] + half;
+            rr[k] = gp[m] <= gg[k] ? m : rr[k];
+        }
+        len -= half;
+    }
+    u32x4 y[NG];
+    uint32_t lo[NG], hi[NG];
+    uint64_t at[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        const uint32_t r = rr[k];
+        const uint32_t p0 = gp[r], p1 = gp[r + 1];
+        const uint64_t ds = desc[r];
+        const uint64_t vb = ds & kStartMask;
+        const uint32_t e3 = uint32_t(ds >> 62);
+        const uint32_t q = gg[k] - p0;
+        const bool live = g0 + uint32_t(k) * 64 + lane <= g1;
+        lo[k] = !live ? 4u : q ? 0u : uint32_t(vb & 3);
+        hi[k] = (gg[k] + 1 == p1 && e3) ? e3 : 4u;
+        at[k] = (vb & ~uint64_t(3)) + 4ull * q;
+        y[k] = ldg(reinterpret_cast<const u32x4*>(col + (vb & ~uint64_t(3))), q);
+    }
+    // filter
+    uint32_t pass = 0;
+    {
+        uint32_t fw[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) fw[e] = rs.fb[filter_bit(y[e >> 2][e & 3], rs.fshift) >> 5];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const uint32_t x = y[e >> 2][e & 3];
+            const uint32_t c = uint32_t(e & 3);
+            const bool ok = c >= lo[e >> 2] && c < hi[e >> 2] && x <= lmax &&
+                            ((fw[e] >> (filter_bit(x, rs.fshift) & 31)) & 1u);
+            pass |= uint32_t(ok) << e;
+        }
+    }
+    // first two hash slots (lanes without a candidate read slot 0: a
+    // broadcast)
+    uint32_t found = 0, cont = 0;
+    uint32_t slot[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const uint32_t x = y[e >> 2][e & 3];
+        const bool pe = (pass >> e) & 1u;
+        const uint32_t h0 = pe ? hash_slot(x, rs.hshift) : 0u;
+        const uint32_t h1 = pe ? (h0 + 1) & rs.hmask : 0u;
+        const uint32_t k0 = rs.hk[h0], k1 = rs.hk[h1];
+        const bool f0 = pe && k0 == x;
+        const bool past0 = pe && k0 != x && k0 != kEmpty;
+        const bool f1 = past0 && k1 == x;
+        found |= uint32_t(f0 || f1) << e;
+        cont |= uint32_t(past0 && k1 != x && k1 != kEmpty) << e;
+        slot[e] = f0 ? h0 : h1;
+    }
+    uint32_t jj[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) jj[e] = kIndex ? rs.hv[((found >> e) & 1u) ? slot[e] : 0u] : 0u;
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+        if ((found >> e) & 1u) hit(rr[e >> 2], jj[e], at[e >> 2] + uint32_t(e & 3));
+    if (cont) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            if ((cont >> e) & 1u) {
+                const uint32_t x = y[e >> 2][e & 3];
+                const uint32_t j = hash_probe(rs.hk, rs.hv, rs.hmask, (slot[e] + 1) & rs.hmask, x);
+                if (j != kNotFound) hit(rr[e >> 2], j, at[e >> 2] + uint32_t(e & 3));
+            }
+        }
+    }
+}
+
 template <int NG = kGroups, bool kIndex = true, class Hit>
 __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigned long long* desc, const uint32_t* gp,
                                               uint32_t nr, uint32_t G, uint32_t tid, uint32_t T, const RootSet& rs,
-                                              uint32_t lmax, Hit&& hit) {
-    constexpr int NE = 4 * NG;
+                                              uint32_t lmax, Hit&& hit, uint32_t* ctr = nullptr) {
+    constexpr uint32_t SPAN = 64 * NG;
     const uint32_t lane = tid & 63;
+    if (ctr) {
+        const uint32_t nspan = (G + SPAN - 1) / SPAN;
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(ctr, 1u);
+        c = __shfl(c, 0);
+        while (c < nspan) {
+            uint32_t cn = 0;
+            if (lane == 0) cn = atomicAdd(ctr, 1u);
+            const uint32_t g0 = c * SPAN, g1 = (g0 + SPAN < G ? g0 + SPAN : G) - 1;
+            const uint32_t rlo = wave_search(gp, 0, nr, g0);
+            const uint32_t rhi = wave_search(gp, rlo, nr, g1) + 1;
+            stream_span<NG, kIndex>(col, desc, gp, g0, g1, rlo, rhi, rs, lmax, hit);
+            c = __shfl(cn, 0);
+        }
+        return;
+    }
     const uint32_t nw = T >> 6;
     const uint32_t per = ((G + nw - 1) / nw + 63) & ~63u;
     const uint32_t wb = (tid >> 6) * per;
@@ -202,92 +315,10 @@ __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigne
     const uint32_t wend = wbeg + per < G ? wbeg + per : G;
     if (wbeg >= wend) return;
     uint32_t rlo = wave_search(gp, 0, nr, wbeg);
-    for (uint32_t g0 = wbeg; g0 < wend; g0 += 64 * NG) {
-        const uint32_t g1 = (g0 + 64 * NG < wend ? g0 + 64 * NG : wend) - 1;
+    for (uint32_t g0 = wbeg; g0 < wend; g0 += SPAN) {
+        const uint32_t g1 = (g0 + SPAN < wend ? g0 + SPAN : wend) - 1;
         const uint32_t rhi = wave_search(gp, rlo, nr, g1) + 1;
-        // rows of this lane's groups (past g1: the group g1, masked below)
-        uint32_t gg[NG], rr[NG];
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            const uint32_t g = g0 + uint32_t(k) * 64 + lane;
-            gg[k] = g < g1 ? g : g1;
-            rr[k] = rlo;
-        }
-        for (uint32_t len = rhi - rlo; len > 1;) {
-            const uint32_t half = len >> 1;
-#pragma unroll
-            for (int k = 0; k < NG; ++k) {
-                const uint32_t m = rr[k] + half;
-                rr[k] = gp[m] <= gg[k] ? m : rr[k];
-            }
-            len -= half;
-        }
-        u32x4 y[NG];
-        uint32_t lo[NG], hi[NG];
-        uint64_t at[NG];
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            const uint32_t r = rr[k];
-            const uint32_t p0 = gp[r], p1 = gp[r + 1];
-            const uint64_t ds = desc[r];
-            const uint64_t vb = ds & kStartMask;
-            const uint32_t e3 = uint32_t(ds >> 62);
-            const uint32_t q = gg[k] - p0;
-            const bool live = g0 + uint32_t(k) * 64 + lane <= g1;
-            lo[k] = !live ? 4u : q ? 0u : uint32_t(vb & 3);
-            hi[k] = (gg[k] + 1 == p1 && e3) ? e3 : 4u;
-            at[k] = (vb & ~uint64_t(3)) + 4ull * q;
-            y[k] = ldg(reinterpret_cast<const u32x4*>(col + (vb & ~uint64_t(3))), q);
-        }
-        // filter
-        uint32_t pass = 0;
-        {
-            uint32_t fw[NE];
-#pragma unroll
-            for (int e = 0; e < NE; ++e) fw[e] = rs.fb[filter_bit(y[e >> 2][e & 3], rs.fshift) >> 5];
-#pragma unroll
-            for (int e = 0; e < NE; ++e) {
-                const uint32_t x = y[e >> 2][e & 3];
-                const uint32_t c = uint32_t(e & 3);
-                const bool ok = c >= lo[e >> 2] && c < hi[e >> 2] && x <= lmax &&
-                                ((fw[e] >> (filter_bit(x, rs.fshift) & 31)) & 1u);
-                pass |= uint32_t(ok) << e;
-            }
-        }
-        // first two hash slots (lanes without a candidate read slot 0: a
-        // broadcast)
-        uint32_t found = 0, cont = 0;
-        uint32_t slot[NE];
-#pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            const uint32_t x = y[e >> 2][e & 3];
-            const bool pe = (pass >> e) & 1u;
-            const uint32_t h0 = pe ? hash_slot(x, rs.hshift) : 0u;
-            const uint32_t h1 = pe ? (h0 + 1) & rs.hmask : 0u;
-            const uint32_t k0 = rs.hk[h0], k1 = rs.hk[h1];
-            const bool f0 = pe && k0 == x;
-            const bool past0 = pe && k0 != x && k0 != kEmpty;
-            const bool f1 = past0 && k1 == x;
-            found |= uint32_t(f0 || f1) << e;
-            cont |= uint32_t(past0 && k1 != x && k1 != kEmpty) << e;
-            slot[e] = f0 ? h0 : h1;
-        }
-        uint32_t jj[NE];
-#pragma unroll
-        for (int e = 0; e < NE; ++e) jj[e] = kIndex ? rs.hv[((found >> e) & 1u) ? slot[e] : 0u] : 0u;
-#pragma unroll
-        for (int e = 0; e < NE; ++e)
-            if ((found >> e) & 1u) hit(rr[e >> 2], jj[e], at[e >> 2] + uint32_t(e & 3));
-        if (cont) {
-#pragma unroll
-            for (int e = 0; e < NE; ++e) {
-                if ((cont >> e) & 1u) {
-                    const uint32_t x = y[e >> 2][e & 3];
-                    const uint32_t j = hash_probe(rs.hk, rs.hv, rs.hmask, (slot[e] + 1) & rs.hmask, x);
-                    if (j != kNotFound) hit(rr[e >> 2], j, at[e >> 2] + uint32_t(e & 3));
-                }
-            }
-        }
+        stream_span<NG, kIndex>(col, desc, gp, g0, g1, rlo, rhi, rs, lmax, hit);
         rlo = rhi - 1;
     }
 }
