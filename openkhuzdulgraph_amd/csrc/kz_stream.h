@@ -332,13 +332,22 @@ __device__ __forceinline__ uint32_t find_global(const uint32_t* __restrict__ a, 
 // Candidates of the skewed rows among rows [0, nr) (list indices r0 + r of
 // L, d ids; candidates of row r are L[r0 + r + 1 .. d)), searched in the
 // rows' lists.  Wave `wave` of nw takes rows r = wave, wave + nw, ..., 64 at a
-// time, and spreads their (row, candidate) pairs over its lanes, NP pairs per
-// lane: the NP binary searches of a lane advance in lockstep, so each step
-// has 64 * NP independent loads in flight (a search is a chain of dependent
-// HBM/L2 loads, the latency is what bounds it).  No workgroup barrier, so it
-// overlaps other waves' streaming.  hit(r, j, p): candidate j found at
-// position p of col.  Call with every lane of the wave.
-template <int NP = 1, class Hit>
+// time.  A search costs about one vector-L1 lookup per distinct cache line
+// it touches (which bounds this phase, not latency: more searches in flight
+// per lane measured slower), so rows are searched two ways:
+//   * rows with fewer than kSampledRow candidates: their (row, candidate)
+//     pairs are spread over the lanes, one binary search each (about
+//     log2(len) - 3 lines);
+//   * the others one at a time: the lanes first read 64 evenly spaced
+//     entries of the row's list (one load), each candidate finds its 1/64 of
+//     the list by a shuffle search over those samples and binary-searches
+//     only that part (64 lines per row, then about 1 - 2 per candidate).
+// No workgroup barrier, so it overlaps other waves' streaming.  hit(r, j, p):
+// candidate j found at position p of col.  Call with every lane of the wave.
+constexpr uint32_t kSampledRow = 16;
+static_assert(kSkewFactor * kSampledRow >= 64, "a sampled row must have more than 64 entries");
+
+template <class Hit>
 __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigned long long* desc, const uint32_t* L,
                                               uint32_t d, uint32_t r0, uint32_t nr, uint32_t wave, uint32_t nw,
                                               Hit&& hit) {
@@ -355,52 +364,78 @@ __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigne
                 cnt = d - 1 - (r0 + r);
             }
         }
+        const bool sampled = cnt >= kSampledRow;
+        // light rows: one (row, candidate) pair per lane
         uint32_t excl;
-        const uint32_t P = wave_exclusive_scan(cnt, &excl);
-        for (uint32_t pbase = 0; pbase < P; pbase += 64 * NP) {
-            uint32_t rr[NP], jj[NP], xx[NP], at[NP], len[NP], v[NP];
-            uint64_t bb[NP];
+        const uint32_t P = wave_exclusive_scan(sampled ? 0u : cnt, &excl);
+        for (uint32_t pbase = 0; pbase < P; pbase += 64) {
+            const uint32_t p = pbase + lane;
+            // owning lane: the largest l with excl(l) <= p
+            uint32_t l = 0;
 #pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                const uint32_t p = pbase + uint32_t(k) * 64 + lane;
-                // owning lane: the largest l with excl(l) <= p
-                uint32_t l = 0;
+            for (uint32_t step = 32; step; step >>= 1) {
+                const uint32_t e = __shfl(excl, int(l + step));
+                if (e <= p) l += step;
+            }
+            const bool live = p < P;
+            const uint32_t rr = __shfl(r, int(l));
+            const uint64_t bb = __shfl(b, int(l));
+            const uint32_t jj = r0 + rr + 1 + (p - __shfl(excl, int(l)));
+            // shuffles run in every lane: a pair's owner may be a lane
+            // without a pair of its own in this round
+            const uint32_t nn = __shfl(n, int(l));
+            const uint32_t x = live ? L[jj] : 0u;
+            uint32_t len = live ? nn : 0u, at = 0;
+            // largest i with a[i] <= x (a = col + bb); v tracks a[at]
+            uint32_t v = ldg(col, bb);
+            while (__ballot(len > 1)) {
+                const uint32_t half = len >> 1;
+                const uint32_t t = ldg(col, bb + at + half);
+                if (half && t <= x) {
+                    at += half;
+                    v = t;
+                }
+                len -= half;
+            }
+            if (live && v == x) hit(rr, jj, bb + at);
+        }
+        // heavy rows: sample, then search one 1/64 of the list
+        unsigned long long hm = __ballot(sampled);
+        while (hm) {
+            const int hl = __builtin_ctzll(hm);
+            hm &= hm - 1;
+            const uint32_t rr = __shfl(r, hl), nn = __shfl(n, hl), cc = __shfl(cnt, hl);
+            const uint64_t bb = __shfl(b, hl);
+            // sample s = a[floor(s * nn / 64)]; nn > 64 (a sampled row has more
+            // than kSkewFactor * kSampledRow entries), so the 64 parts are non-empty
+            const uint32_t smp = ldg(col, bb + ((uint64_t(lane) * nn) >> 6));
+            for (uint32_t cb = 0; cb < cc; cb += 64) {
+                const uint32_t c = cb + lane;
+                const bool live = c < cc;
+                const uint32_t jj = r0 + rr + 1 + c;
+                const uint32_t x = live ? L[jj] : 0u;
+                // the part: the largest s with sample s <= x (s = 0 if none)
+                uint32_t sp = 0;
 #pragma unroll
                 for (uint32_t step = 32; step; step >>= 1) {
-                    const uint32_t e = __shfl(excl, int(l + step));
-                    if (e <= p) l += step;
+                    const uint32_t e = __shfl(smp, int(sp + step));
+                    if (e <= x) sp += step;
                 }
-                const bool live = p < P;
-                rr[k] = __shfl(r, int(l));
-                bb[k] = __shfl(b, int(l));
-                const uint32_t nn = __shfl(n, int(l));
-                jj[k] = r0 + rr[k] + 1 + (p - __shfl(excl, int(l)));
-                xx[k] = live ? L[jj[k]] : 0u;
-                len[k] = live ? nn : 0u;
-                at[k] = 0;
-            }
-            // largest i with a[i] <= x (a = col + bb); v tracks a[at]
-#pragma unroll
-            for (int k = 0; k < NP; ++k) v[k] = ldg(col, bb[k]);
-            while (true) {
-                bool more = false;
-#pragma unroll
-                for (int k = 0; k < NP; ++k) more |= len[k] > 1;
-                if (!__ballot(more)) break;
-#pragma unroll
-                for (int k = 0; k < NP; ++k) {
-                    const uint32_t half = len[k] >> 1;
-                    const uint32_t t = ldg(col, bb[k] + at[k] + half);
-                    if (half && t <= xx[k]) {
-                        at[k] += half;
-                        v[k] = t;
+                uint32_t at = uint32_t((uint64_t(sp) * nn) >> 6);
+                const uint32_t hi = sp == 63 ? nn : uint32_t((uint64_t(sp + 1) * nn) >> 6);
+                uint32_t len = hi - at;
+                uint32_t v = __shfl(smp, int(sp));
+                while (__ballot(len > 1)) {
+                    const uint32_t half = len >> 1;
+                    const uint32_t t = ldg(col, bb + at + half);
+                    if (half && t <= x) {
+                        at += half;
+                        v = t;
                     }
-                    len[k] -= half;
+                    len -= half;
                 }
+                if (live && v == x) hit(rr, jj, bb + at);
             }
-#pragma unroll
-            for (int k = 0; k < NP; ++k)
-                if (pbase + uint32_t(k) * 64 + lane < P && v[k] == xx[k]) hit(rr[k], jj[k], bb[k] + at[k]);
         }
     }
 }
