@@ -120,7 +120,14 @@ struct RootSet {
     uint32_t fshift;
 };
 
-__device__ __forceinline__ uint32_t filter_bit(uint32_t x, uint32_t fshift) { return (x * 0x85EBCA6Bu) >> fshift; }
+// Filter bit of x: the bits just below the hash slot's in the same
+// Fibonacci product (hash_slot), so one 32-bit multiply serves both.
+__device__ __forceinline__ uint32_t filter_bit_of(uint32_t prod, uint32_t hshift, uint32_t fshift) {
+    return (prod << (32 - hshift)) >> fshift;
+}
+__device__ __forceinline__ uint32_t filter_bit(uint32_t x, uint32_t hshift, uint32_t fshift) {
+    return filter_bit_of(x * 2654435761u, hshift, fshift);
+}
 
 // Clears the hash (H slots) and filter (fw words) over threads tid..T.
 __device__ __forceinline__ void root_clear(const RootSet& s, uint32_t H, uint32_t fw, uint32_t tid, uint32_t T) {
@@ -130,12 +137,12 @@ __device__ __forceinline__ void root_clear(const RootSet& s, uint32_t H, uint32_
 
 __device__ __forceinline__ void root_insert(const RootSet& s, uint32_t x, uint16_t idx) {
     hash_insert(s.hk, s.hv, s.hmask, s.hshift, x, idx);
-    const uint32_t b = filter_bit(x, s.fshift);
+    const uint32_t b = filter_bit(x, s.hshift, s.fshift);
     atomicOr(&s.fb[b >> 5], 1u << (b & 31));
 }
 
 __device__ __forceinline__ uint32_t root_find(const RootSet& s, uint32_t x) {
-    const uint32_t b = filter_bit(x, s.fshift);
+    const uint32_t b = filter_bit(x, s.hshift, s.fshift);
     if (!((s.fb[b >> 5] >> (b & 31)) & 1u)) return kNotFound;
     return hash_find(s.hk, s.hv, s.hmask, s.hshift, x);
 }
@@ -235,19 +242,24 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
         at[k] = (vb & ~uint64_t(3)) + 4ull * q;
         y[k] = ldg(reinterpret_cast<const u32x4*>(col + (vb & ~uint64_t(3))), q);
     }
-    // filter
+    // filter (one multiply per id gives the filter bit and the hash slot;
+    // the mask is built without branches)
     uint32_t pass = 0;
+    uint32_t prod[NE];
     {
         uint32_t fw[NE];
 #pragma unroll
-        for (int e = 0; e < NE; ++e) fw[e] = rs.fb[filter_bit(y[e >> 2][e & 3], rs.fshift) >> 5];
+        for (int e = 0; e < NE; ++e) {
+            prod[e] = y[e >> 2][e & 3] * 2654435761u;
+            fw[e] = rs.fb[filter_bit_of(prod[e], rs.hshift, rs.fshift) >> 5];
+        }
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const uint32_t x = y[e >> 2][e & 3];
             const uint32_t c = uint32_t(e & 3);
-            const bool ok = c >= lo[e >> 2] && c < hi[e >> 2] && x <= lmax &&
-                            ((fw[e] >> (filter_bit(x, rs.fshift) & 31)) & 1u);
-            pass |= uint32_t(ok) << e;
+            const uint32_t ok = uint32_t(c >= lo[e >> 2]) & uint32_t(c < hi[e >> 2]) & uint32_t(x <= lmax) &
+                                (fw[e] >> (filter_bit_of(prod[e], rs.hshift, rs.fshift) & 31));
+            pass |= (ok & 1u) << e;
         }
     }
     // first two hash slots (lanes without a candidate read slot 0: a
@@ -258,7 +270,7 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
     for (int e = 0; e < NE; ++e) {
         const uint32_t x = y[e >> 2][e & 3];
         const bool pe = (pass >> e) & 1u;
-        const uint32_t h0 = pe ? hash_slot(x, rs.hshift) : 0u;
+        const uint32_t h0 = pe ? prod[e] >> rs.hshift : 0u;
         const uint32_t h1 = pe ? (h0 + 1) & rs.hmask : 0u;
         const uint32_t k0 = rs.hk[h0], k1 = rs.hk[h1];
         const bool f0 = pe && k0 == x;
