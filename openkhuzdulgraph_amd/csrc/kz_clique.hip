@@ -62,6 +62,11 @@ struct HubMeta {
 };
 
 // ------------------------------------------------------------ wave tasks --
+// Roots per cursor atomic of the wave-task kernel (measured on RMAT-22:
+// chunks shrinking to one root at the end of the list 39.0 ms; fixed chunks
+// of 4 / 8 / 16 / 32: 37.6 / 37.0 / 36.9 / 36.8 ms).
+constexpr uint32_t kWaveTaskChunk = 32;
+
 // One task per wave, |N+(u)| <= 64.  LDS per wave: L[64], 128-slot hash,
 // 64 bitmap rows (one u64 each) and the row descriptors.  Tasks are small,
 // so the chain cursor -> root -> list bounds of the NEXT task is issued
@@ -94,7 +99,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void c
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
     const uint32_t workers = gridDim.x * WPB;
-    TaskChunk chunk(ntasks, workers, 2);
+    TaskChunk chunk(ntasks, workers, 2, kWaveTaskChunk, kWaveTaskChunk);
     auto fetch = [&]() { return wave_next_task(chunk, cursor); };
     // current task: index, query, list start and length, graph arrays
     uint32_t t = fetch(), q = 0, d = 0;

@@ -56,30 +56,34 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t* keys, const uint16
 // reserves `size` consecutive tasks.  Tasks come largest first, so the size
 // ramps up with the tasks already taken (at most base / (ramp * workers))
 // and shrinks again with the tasks left (at most an eighth of the caller's
-// share of them), within [1, kMaxTaskChunk]: the heavy first tasks still go
-// one at a time, the many light ones in chunks.  A slow ramp (8) suits
-// kernels whose task costs vary by orders of magnitude (workgroup tasks),
-// a fast one (2) kernels of small, similar tasks (wave tasks).  Each cursor
+// share of them), within [floor, cap]: the heavy first tasks still go one at
+// a time, the many light ones in chunks.  A slow ramp (8), floor 1 and cap
+// kMaxTaskChunk suit kernels whose task costs vary by orders of magnitude
+// (workgroup tasks); kernels of small, similar tasks (one root per wave)
+// take fixed chunks (floor = cap), since their last tasks would otherwise
+// queue on the cursor one atomic each.  Each cursor
 // is the first u32 of its own kCursorStride-word (256-byte) line: atomics on
 // one line are serialised at one L2 channel, on the order of 10^8 per
 // second, which bounds a kernel of short tasks taking one per atomic.
 constexpr uint32_t kCursorStride = 64;
 constexpr uint32_t kMaxTaskChunk = 32;
 
-__device__ __forceinline__ uint32_t task_chunk(uint32_t base, uint32_t ntasks, uint32_t workers, uint32_t ramp) {
+__device__ __forceinline__ uint32_t task_chunk(uint32_t base, uint32_t ntasks, uint32_t workers, uint32_t ramp,
+                                               uint32_t floor, uint32_t cap) {
     const uint32_t left = base < ntasks ? ntasks - base : 0u;
     const uint32_t a = left / (8u * workers), b = base / (ramp * workers);
     const uint32_t c = a < b ? a : b;
-    return c < 1u ? 1u : (c > kMaxTaskChunk ? kMaxTaskChunk : c);
+    return c < floor ? floor : (c > cap ? cap : c);
 }
 
 struct TaskChunk {
-    uint32_t next, end, size, ntasks, workers, ramp;
-    __device__ TaskChunk(uint32_t n, uint32_t w, uint32_t r) : next(0), end(0), size(1), ntasks(n), workers(w), ramp(r) {}
+    uint32_t next, end, size, ntasks, workers, ramp, floor, cap;
+    __device__ TaskChunk(uint32_t n, uint32_t w, uint32_t r, uint32_t f = 1, uint32_t c = kMaxTaskChunk)
+        : next(0), end(0), size(f), ntasks(n), workers(w), ramp(r), floor(f), cap(c) {}
     __device__ void take(uint32_t base) {
         next = base;
         end = base + size;
-        size = task_chunk(end, ntasks, workers, ramp);
+        size = task_chunk(end, ntasks, workers, ramp, floor, cap);
     }
 };
 
