@@ -131,8 +131,8 @@ __global__ void fill_small_kernel(uint32_t* __restrict__ a, int na, unsigned lon
 
 // Pass 1: triangles per forward edge.  One root per workgroup of T threads
 // (T = 64: roots with d+ <= 64, one wave).  Dynamic LDS: [row descriptors
-// Dmax u64][L Dmax u32][edge counters Dmax u32][group prefix Dmax+1 u32]
-// [filter FW u32][hash keys H u32][hash vals H u16].
+// Dmax u64][hash keys H u32][L Dmax u32][edge counters Dmax u32][group
+// prefix Dmax+1 u32][filter FW u32][hash vals H u16].
 template <int T>
 __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restrict__ frowptr,
                                                        const uint32_t* __restrict__ fcol,
@@ -144,6 +144,8 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
     unsigned char* p = smem;
     unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
     p += size_t(Dmax) * 8;
+    uint32_t* hk = reinterpret_cast<uint32_t*>(p);  // 8-byte aligned (pair reads)
+    p += size_t(H) * 4;
     uint32_t* L = reinterpret_cast<uint32_t*>(p);
     p += size_t(Dmax) * 4;
     uint32_t* cnt = reinterpret_cast<uint32_t*>(p);
@@ -152,8 +154,6 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
     p += size_t(Dmax + 1) * 4;
     uint32_t* fb = reinterpret_cast<uint32_t*>(p);
     p += size_t(FW) * 4;
-    uint32_t* hk = reinterpret_cast<uint32_t*>(p);
-    p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
     __shared__ uint32_t sTask, sTmp[T / 64], sSpan;
     const RootSet rs{hk, hv, H - 1, hshift, fb, fshift};

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void c
     constexpr int WPB = 4;
     constexpr uint32_t H = 128;
     __shared__ uint32_t sL[WPB][64];
-    __shared__ uint32_t sKey[WPB][H];
+    __shared__ __align__(8) uint32_t sKey[WPB][H];
     __shared__ uint16_t sVal[WPB][H];
     __shared__ uint32_t sFb[WPB][32];  // 1024-bit filter: 16 bits per id
     __shared__ unsigned long long sRow[WPB][64];
@@ -344,21 +344,21 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
         col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
         [&](uint32_t r, uint32_t j, uint64_t) {
             if (K == 3) acc += 1;
-            else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
+            else atomicOr(&rows_out[__umul24(j >> 6, ws) + r], 1ull << (j & 63));
         },
         ctr);
     // skewed rows: each wave searches the candidates of its own rows
     search_skewed(col, rb, L, d, r0, nr, threadIdx.x >> 6, blockDim.x >> 6, [&](uint32_t r, uint32_t j, uint64_t) {
         if (K == 3) acc += 1;
-        else atomicOr(&rows_out[size_t(j >> 6) * ws + r], 1ull << (j & 63));
+        else atomicOr(&rows_out[__umul24(j >> 6, ws) + r], 1ull << (j & 63));
     });
     __syncthreads();
 }
 
 // ----------------------------------------------------------- block tasks --
 // One task per workgroup, rows in LDS.  Dynamic LDS: [rows Dmax*W u64 (k>=4)]
-// [row descriptors Dmax u64][L Dmax u32][group prefix Dmax+1 u32]
-// [filter FW u32][hash keys H u32][hash vals H u16].  Register budget: 6
+// [row descriptors Dmax u64][hash keys H u32][L Dmax u32][group prefix
+// Dmax+1 u32][filter FW u32][hash vals H u16].  Register budget: 6
 // waves per SIMD (at most 80 VGPRs), the occupancy the LDS of the middle
 // buckets allows; the latency-bound count phase loses more to fewer waves
 // than the stream gains from more registers.
@@ -379,14 +379,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void 
     }
     unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
     p += size_t(Dmax) * 8;
+    uint32_t* hk = reinterpret_cast<uint32_t*>(p);  // 8-byte aligned (pair reads)
+    p += size_t(H) * 4;
     uint32_t* L = reinterpret_cast<uint32_t*>(p);
     p += size_t(Dmax) * 4;
     uint32_t* rp = reinterpret_cast<uint32_t*>(p);
     p += size_t(Dmax + 1) * 4;
     uint32_t* fb = reinterpret_cast<uint32_t*>(p);
     p += size_t(FW) * 4;
-    uint32_t* hk = reinterpret_cast<uint32_t*>(p);
-    p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
     __shared__ uint32_t sTask, sTmp[16], sSpan;
     const int wave = threadIdx.x >> 6;
@@ -431,8 +431,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void 
 // Hub roots are split: item c of hub h builds rows [c*RPC, (c+1)*RPC) in an
 // LDS buffer and stores them to the hub's HBM slab (k >= 4), or counts the
 // matches directly (k = 3).  Dynamic LDS: [row buffer RPC*Wmax u64 (k>=4)]
-// [row descriptors RPC u64][L Dmax u32][group prefix RPC+1 u32]
-// [hash keys H u32][hash vals H u16].
+// [row descriptors RPC u64][hash keys H u32][L Dmax u32][group prefix
+// RPC+1 u32][filter FW u32][hash vals H u16].
 template <int K>
 __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restrict__ queries,
                                                        const HubMeta* __restrict__ hubs, uint32_t nhubs,
@@ -450,14 +450,14 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
     }
     unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
     p += size_t(RPC) * 8;
+    uint32_t* hk = reinterpret_cast<uint32_t*>(p);  // 8-byte aligned (pair reads)
+    p += size_t(H) * 4;
     uint32_t* L = reinterpret_cast<uint32_t*>(p);
     p += size_t(Dmax) * 4;
     uint32_t* rp = reinterpret_cast<uint32_t*>(p);
     p += size_t(RPC + 1) * 4;
     uint32_t* fb = reinterpret_cast<uint32_t*>(p);
     p += size_t(FW) * 4;
-    uint32_t* hk = reinterpret_cast<uint32_t*>(p);
-    p += size_t(H) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
     __shared__ uint32_t sItem, sTmp[16], sSpan;
     unsigned long long acc = 0;

@@ -112,6 +112,9 @@ __device__ __forceinline__ bool row_desc(const uint64_t* rowptr, uint32_t v, uin
 // The root's list in LDS: an open-addressing hash (id -> index) and a
 // one-hash membership filter of 2^(32 - fshift) bits (at least 16 per key)
 // that rejects most non-members with a single LDS read before the probe.
+// The hash probes in pairs: a key's first two slots are its home slot h and
+// h ^ 1, read together with one 8-byte load (hk is 8-byte aligned), then
+// (h | 1) + 1, ... linearly.
 struct RootSet {
     uint32_t* hk;
     uint16_t* hv;
@@ -120,13 +123,24 @@ struct RootSet {
     uint32_t fshift;
 };
 
-// Filter bit of x: the bits just below the hash slot's in the same
-// Fibonacci product (hash_slot), so one 32-bit multiply serves both.
+// Filter bit of x: the 32 - fshift bits just below the hash slot's in the
+// same Fibonacci product (hash_slot), so one 32-bit multiply serves both
+// (a filter wider than the bits below the slot uses those bits only).  The
+// bit starts at bit filter_offset of the product.
+__device__ __forceinline__ uint32_t filter_offset(uint32_t hshift, uint32_t fshift) {
+    const uint32_t fbits = 32 - fshift;
+    return fbits < hshift ? hshift - fbits : 0u;
+}
 __device__ __forceinline__ uint32_t filter_bit_of(uint32_t prod, uint32_t hshift, uint32_t fshift) {
-    return (prod << (32 - hshift)) >> fshift;
+    const uint32_t off = filter_offset(hshift, fshift);
+    return __builtin_amdgcn_ubfe(prod, off, hshift - off);
 }
 __device__ __forceinline__ uint32_t filter_bit(uint32_t x, uint32_t hshift, uint32_t fshift) {
     return filter_bit_of(x * 2654435761u, hshift, fshift);
+}
+// Bit b of the filter, given its word.
+__device__ __forceinline__ uint32_t filter_has(uint32_t word, uint32_t b) {
+    return __builtin_amdgcn_ubfe(word, b & 31u, 1u);
 }
 
 // Clears the hash (H slots) and filter (fw words) over threads tid..T.
@@ -135,16 +149,21 @@ __device__ __forceinline__ void root_clear(const RootSet& s, uint32_t H, uint32_
     for (uint32_t i = tid; i < fw; i += T) s.fb[i] = 0;
 }
 
+// Inserts x at the first free slot of its probe order (h, h ^ 1, then
+// (h | 1) + 1 onwards); concurrent inserts of distinct keys are safe.
 __device__ __forceinline__ void root_insert(const RootSet& s, uint32_t x, uint16_t idx) {
-    hash_insert(s.hk, s.hv, s.hmask, s.hshift, x, idx);
+    const uint32_t h0 = hash_slot(x, s.hshift);
+    uint32_t h = h0;
+    if (atomicCAS(&s.hk[h], kEmpty, x) != kEmpty) {
+        h = h0 ^ 1u;
+        if (atomicCAS(&s.hk[h], kEmpty, x) != kEmpty) {
+            h = ((h0 | 1u) + 1u) & s.hmask;
+            while (atomicCAS(&s.hk[h], kEmpty, x) != kEmpty) h = (h + 1) & s.hmask;
+        }
+    }
+    s.hv[h] = idx;
     const uint32_t b = filter_bit(x, s.hshift, s.fshift);
     atomicOr(&s.fb[b >> 5], 1u << (b & 31));
-}
-
-__device__ __forceinline__ uint32_t root_find(const RootSet& s, uint32_t x) {
-    const uint32_t b = filter_bit(x, s.hshift, s.fshift);
-    if (!((s.fb[b >> 5] >> (b & 31)) & 1u)) return kNotFound;
-    return hash_find(s.hk, s.hv, s.hmask, s.hshift, x);
 }
 
 // Filter geometry for lists of up to dmax ids: words and shift.
@@ -253,32 +272,31 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
             prod[e] = y[e >> 2][e & 3] * 2654435761u;
             fw[e] = rs.fb[filter_bit_of(prod[e], rs.hshift, rs.fshift) >> 5];
         }
+        // the bit within its word: the low 5 bits of the filter bit
+        const uint32_t foff = filter_offset(rs.hshift, rs.fshift);
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const uint32_t x = y[e >> 2][e & 3];
             const uint32_t c = uint32_t(e & 3);
             const uint32_t ok = uint32_t(c >= lo[e >> 2]) & uint32_t(c < hi[e >> 2]) & uint32_t(x <= lmax) &
-                                (fw[e] >> (filter_bit_of(prod[e], rs.hshift, rs.fshift) & 31));
-            pass |= (ok & 1u) << e;
+                                filter_has(fw[e], prod[e] >> foff);
+            pass |= ok << e;
         }
     }
-    // first two hash slots (lanes without a candidate read slot 0: a
-    // broadcast)
+    // home pairs (h & ~1, h | 1): one 8-byte read each (lanes without a
+    // candidate read pair 0: a broadcast)
     uint32_t found = 0, cont = 0;
     uint32_t slot[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
         const uint32_t x = y[e >> 2][e & 3];
         const bool pe = (pass >> e) & 1u;
-        const uint32_t h0 = pe ? prod[e] >> rs.hshift : 0u;
-        const uint32_t h1 = pe ? (h0 + 1) & rs.hmask : 0u;
-        const uint32_t k0 = rs.hk[h0], k1 = rs.hk[h1];
-        const bool f0 = pe && k0 == x;
-        const bool past0 = pe && k0 != x && k0 != kEmpty;
-        const bool f1 = past0 && k1 == x;
-        found |= uint32_t(f0 || f1) << e;
-        cont |= uint32_t(past0 && k1 != x && k1 != kEmpty) << e;
-        slot[e] = f0 ? h0 : h1;
+        const uint32_t hp = pe ? prod[e] >> (rs.hshift + 1) : 0u;
+        const uint2 kp = reinterpret_cast<const uint2*>(rs.hk)[hp];
+        const bool f0 = kp.x == x, f1 = kp.y == x;
+        found |= uint32_t(pe && (f0 || f1)) << e;
+        cont |= uint32_t(pe && !f0 && !f1 && kp.x != kEmpty && kp.y != kEmpty) << e;
+        slot[e] = 2 * hp + uint32_t(f1);
     }
     uint32_t jj[NE];
 #pragma unroll
@@ -291,7 +309,7 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
         for (int e = 0; e < NE; ++e) {
             if ((cont >> e) & 1u) {
                 const uint32_t x = y[e >> 2][e & 3];
-                const uint32_t j = hash_probe(rs.hk, rs.hv, rs.hmask, (slot[e] + 1) & rs.hmask, x);
+                const uint32_t j = hash_probe(rs.hk, rs.hv, rs.hmask, (slot[e] + 2) & rs.hmask, x);
                 if (j != kNotFound) hit(rr[e >> 2], j, at[e >> 2] + uint32_t(e & 3));
             }
         }
