@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void c
     constexpr uint32_t H = 128;
     __shared__ uint32_t sL[WPB][64];
     __shared__ __align__(8) uint32_t sKey[WPB][H];
-    __shared__ uint16_t sVal[WPB][H];
+    __shared__ __align__(4) uint16_t sVal[WPB][H];
     __shared__ uint32_t sFb[WPB][32];  // 1024-bit filter: 16 bits per id
     __shared__ unsigned long long sRow[WPB][64];
     __shared__ unsigned long long sRb[WPB][64];
@@ -316,6 +316,16 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
     }
 }
 
+// Sets bit j of row r in a word-major LDS bitmap (word kk of row r at
+// rows[kk * ws + r], ws < 2^24).  The word index is one v_mad_u32_u24
+// (written out: the compiler otherwise distributes the byte scaling over
+// the multiply and the add).
+__device__ __forceinline__ void set_row_bit(unsigned long long* rows, uint32_t ws, uint32_t r, uint32_t j) {
+    uint32_t w;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(w) : "v"(j >> 6), "s"(ws), "v"(r));
+    atomicOr(rows + w, 1ull << (j & 63));
+}
+
 // Builds rows [r0, r1) of the DAG of root with list L (d ids, set rs) into
 // rows_out (word kk of row r at rows_out[kk * ws + r - r0], zeroed by the
 // caller), streaming the neighbour lists of all rows as 16-byte groups.  rb holds
@@ -344,13 +354,13 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
         col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
         [&](uint32_t r, uint32_t j, uint64_t) {
             if (K == 3) acc += 1;
-            else atomicOr(&rows_out[__umul24(j >> 6, ws) + r], 1ull << (j & 63));
+            else set_row_bit(rows_out, ws, r, j);
         },
         ctr);
     // skewed rows: each wave searches the candidates of its own rows
     search_skewed(col, rb, L, d, r0, nr, threadIdx.x >> 6, blockDim.x >> 6, [&](uint32_t r, uint32_t j, uint64_t) {
         if (K == 3) acc += 1;
-        else atomicOr(&rows_out[__umul24(j >> 6, ws) + r], 1ull << (j & 63));
+        else set_row_bit(rows_out, ws, r, j);
     });
     __syncthreads();
 }

@@ -113,7 +113,8 @@ __device__ __forceinline__ bool row_desc(const uint64_t* rowptr, uint32_t v, uin
 // one-hash membership filter of 2^(32 - fshift) bits (at least 16 per key)
 // that rejects most non-members with a single LDS read before the probe.
 // The hash probes in pairs: a key's first two slots are its home slot h and
-// h ^ 1, read together with one 8-byte load (hk is 8-byte aligned), then
+// h ^ 1, read together with one 8-byte load (hk is 8-byte aligned, hv
+// 4-byte aligned so that a pair's two indices are one 4-byte load), then
 // (h | 1) + 1, ... linearly.
 struct RootSet {
     uint32_t* hk;
@@ -262,54 +263,53 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
         y[k] = ldg(reinterpret_cast<const u32x4*>(col + (vb & ~uint64_t(3))), q);
     }
     // filter (one multiply per id gives the filter bit and the hash slot;
-    // the mask is built without branches)
-    uint32_t pass = 0;
+    // the conditions are built without branches)
+    bool pass[NE];
     uint32_t prod[NE];
     {
+        const uint32_t foff = filter_offset(rs.hshift, rs.fshift);
+        const uint32_t fwid = rs.hshift - foff;  // filter bits, at least 6
         uint32_t fw[NE];
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             prod[e] = y[e >> 2][e & 3] * 2654435761u;
-            fw[e] = rs.fb[filter_bit_of(prod[e], rs.hshift, rs.fshift) >> 5];
+            fw[e] = rs.fb[__builtin_amdgcn_ubfe(prod[e], foff + 5, fwid - 5)];
         }
-        // the bit within its word: the low 5 bits of the filter bit
-        const uint32_t foff = filter_offset(rs.hshift, rs.fshift);
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const uint32_t x = y[e >> 2][e & 3];
             const uint32_t c = uint32_t(e & 3);
-            const uint32_t ok = uint32_t(c >= lo[e >> 2]) & uint32_t(c < hi[e >> 2]) & uint32_t(x <= lmax) &
-                                filter_has(fw[e], prod[e] >> foff);
-            pass |= ok << e;
+            pass[e] = (c >= lo[e >> 2]) & (c < hi[e >> 2]) & (x <= lmax) & (filter_has(fw[e], prod[e] >> foff) != 0u);
         }
     }
-    // home pairs (h & ~1, h | 1): one 8-byte read each (lanes without a
-    // candidate read pair 0: a broadcast)
-    uint32_t found = 0, cont = 0;
-    uint32_t slot[NE];
+    // home pairs (h & ~1, h | 1): keys with one 8-byte read, their indices
+    // with one 4-byte read beside it (lanes without a candidate read pair 0:
+    // a broadcast)
+    bool found[NE], cont[NE];
+    uint32_t hp[NE], jj[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
         const uint32_t x = y[e >> 2][e & 3];
-        const bool pe = (pass >> e) & 1u;
-        const uint32_t hp = pe ? prod[e] >> (rs.hshift + 1) : 0u;
-        const uint2 kp = reinterpret_cast<const uint2*>(rs.hk)[hp];
+        hp[e] = pass[e] ? prod[e] >> (rs.hshift + 1) : 0u;
+        const uint2 kp = reinterpret_cast<const uint2*>(rs.hk)[hp[e]];
+        const uint32_t vp = kIndex ? reinterpret_cast<const uint32_t*>(rs.hv)[hp[e]] : 0u;
         const bool f0 = kp.x == x, f1 = kp.y == x;
-        found |= uint32_t(pe && (f0 || f1)) << e;
-        cont |= uint32_t(pe && !f0 && !f1 && kp.x != kEmpty && kp.y != kEmpty) << e;
-        slot[e] = 2 * hp + uint32_t(f1);
+        found[e] = pass[e] & (f0 | f1);
+        cont[e] = pass[e] & !f0 & !f1 & (kp.x != kEmpty) & (kp.y != kEmpty);
+        jj[e] = f1 ? vp >> 16 : vp & 0xFFFFu;
     }
-    uint32_t jj[NE];
+    bool any_cont = false;
 #pragma unroll
-    for (int e = 0; e < NE; ++e) jj[e] = kIndex ? rs.hv[((found >> e) & 1u) ? slot[e] : 0u] : 0u;
-#pragma unroll
-    for (int e = 0; e < NE; ++e)
-        if ((found >> e) & 1u) hit(rr[e >> 2], jj[e], at[e >> 2] + uint32_t(e & 3));
-    if (cont) {
+    for (int e = 0; e < NE; ++e) {
+        if (found[e]) hit(rr[e >> 2], jj[e], at[e >> 2] + uint32_t(e & 3));
+        any_cont |= cont[e];
+    }
+    if (any_cont) {
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            if ((cont >> e) & 1u) {
+            if (cont[e]) {
                 const uint32_t x = y[e >> 2][e & 3];
-                const uint32_t j = hash_probe(rs.hk, rs.hv, rs.hmask, (slot[e] + 2) & rs.hmask, x);
+                const uint32_t j = hash_probe(rs.hk, rs.hv, rs.hmask, (2 * hp[e] + 2) & rs.hmask, x);
                 if (j != kNotFound) hit(rr[e >> 2], j, at[e >> 2] + uint32_t(e & 3));
             }
         }
