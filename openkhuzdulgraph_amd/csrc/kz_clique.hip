@@ -331,8 +331,8 @@ __device__ __forceinline__ void set_row_bit(unsigned long long* rows, uint32_t w
 // caller), streaming the neighbour lists of all rows as 16-byte groups.  rb holds
 // (r1 - r0) row descriptors and rp (r1 - r0 + 1) group prefixes, both LDS
 // scratch; tmp holds one u32 per wave and ctr one u32 (the stream's span
-// counter).  Block-wide.
-template <int K>
+// counter).  Block-wide; NG groups of 4 entries per lane and span.
+template <int K, int NG = kGroups>
 __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
                                                  const uint32_t* L, uint32_t d, const RootSet& rs, uint32_t r0, uint32_t r1,
                                                  unsigned long long* rows_out, uint32_t ws, unsigned long long* rb,
@@ -350,7 +350,7 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
     }
     __syncthreads();
     const uint32_t G = block_exclusive_scan(rp, nr + 1, tmp);
-    stream_groups<kGroups, (K >= 4)>(
+    stream_groups<NG, (K >= 4)>(
         col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
         [&](uint32_t r, uint32_t j, uint64_t) {
             if (K == 3) acc += 1;
@@ -368,12 +368,15 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
 // ----------------------------------------------------------- block tasks --
 // One task per workgroup, rows in LDS.  Dynamic LDS: [rows Dmax*W u64 (k>=4)]
 // [row descriptors Dmax u64][hash keys H u32][L Dmax u32][group prefix
-// Dmax+1 u32][filter FW u32][hash vals H u16].  Register budget: 6
-// waves per SIMD (at most 80 VGPRs), the occupancy the LDS of the middle
-// buckets allows; the latency-bound count phase loses more to fewer waves
-// than the stream gains from more registers.
-template <int K>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void clique_block_kernel(const QueryRef* __restrict__ queries,
+// Dmax+1 u32][filter FW u32][hash vals H u16].  Register budget: WPE
+// waves per SIMD.  The default, 6 (at most 80 VGPRs), is the occupancy the
+// LDS of the middle buckets allows; there the latency-bound count phase
+// loses more to fewer waves than the stream gains from more registers.
+// Buckets whose LDS leaves at most 4 waves per SIMD (4-cliques on the
+// 512-1024 bucket) use WPE = 4 and stream NG = 3 groups per lane and span
+// with the registers that frees (RMAT-22 4-clique count 34.97 -> 34.76 ms).
+template <int K, int NG = kGroups, int WPE = 6>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) void clique_block_kernel(const QueryRef* __restrict__ queries,
                                                            const uint32_t* __restrict__ task_v,
                                                            const uint32_t* __restrict__ task_q, uint32_t ntasks,
                                                            uint32_t* __restrict__ cursor,
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void 
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
         __syncthreads();
-        build_rows_block<K>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc);
+        build_rows_block<K, NG>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc);
         if (K == 4) count_rows_mfma4(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
         else if (K == 5) count_rows<K>(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
         __syncthreads();
@@ -634,6 +637,7 @@ static int set_lds_attrs() {
     const void* fns[] = {reinterpret_cast<const void*>(&clique_block_kernel<3>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4>),
                          reinterpret_cast<const void*>(&clique_block_kernel<5>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4, 3, 4>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<3>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<4>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<5>)};
@@ -682,6 +686,13 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         const uint32_t grid = std::min<uint32_t>(nt, uint32_t(R.num_cus) * per_cu);
         uint32_t FW, fshift;
         filter_geometry(Dmax, &FW, &fshift);
+        if constexpr (K == 4) {
+            if (per_cu * block / 64 <= 16) {  // at most 4 waves per SIMD
+                hipLaunchKernelGGL((clique_block_kernel<4, 3, 4>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
+                                   cur, x.d_result, Dmax, W, H, hshift, FW, fshift);
+                return KZ_OK;
+            }
+        }
         hipLaunchKernelGGL(clique_block_kernel<K>, dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt, cur,
                            x.d_result, Dmax, W, H, hshift, FW, fshift);
         return KZ_OK;
