@@ -205,12 +205,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void c
 // (wave-cooperative: lanes take the set bits j of row i).  The bitmap is
 // word-major: word kk of row i is rows[kk * ws + i], so the lanes' rows j
 // sit in consecutive words (no LDS bank conflicts) and row i is a broadcast.
+// With ctr (an LDS counter zeroed before a barrier) rows are taken one
+// at a time from it instead (first and stride unused): rows come largest
+// first, so the waves finish together.
 template <int K>
 __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t first,
-                                           uint32_t stride, unsigned long long& acc) {
+                                           uint32_t stride, unsigned long long& acc, uint32_t* ctr = nullptr) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wd = (d + 63) >> 6;
-    for (uint32_t i = first; i + 2 < d; i += stride) {
+    auto next = [&](uint32_t i) -> uint32_t {
+        if (!ctr) return i + stride;
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(ctr, 1u);
+        return __shfl(t, 0);
+    };
+    for (uint32_t i = ctr ? next(0) : first; i + 2 < d; i = next(i)) {
         const unsigned long long* ri = rows + i;
         const uint32_t k_lo = (i + 1) >> 6;
         uint32_t total = 0;
@@ -269,14 +278,28 @@ __device__ __forceinline__ kz_v4i bits_to_bytes(uint32_t f) {
     return r;
 }
 
+//
+// Tiles go to the workgroup's waves from a counter in LDS (ctr, zeroed
+// before a barrier), the next tile's atomic issued ahead: tile t = J (J + 1)
+// / 2 + I, so the costly tiles (small J: the most 64-bit words) come first
+// and the waves finish together.
 __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows, uint32_t ws, uint32_t d,
-                                                 uint32_t wave, uint32_t nwaves, unsigned long long& acc) {
+                                                 uint32_t* ctr, unsigned long long& acc) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nb = (d + 31) >> 5;
+    const uint32_t ntiles = nb * (nb + 1) / 2;
     uint32_t tile = 0;
-    for (uint32_t J = 0; J < nb; ++J) {
-        for (uint32_t I = 0; I <= J; ++I, ++tile) {
-            if (tile % nwaves != wave) continue;
+    if (lane == 0) tile = atomicAdd(ctr, 1u);
+    tile = __shfl(tile, 0);
+    while (tile < ntiles) {
+        uint32_t tn = 0;
+        if (lane == 0) tn = atomicAdd(ctr, 1u);
+        // J: the largest with J (J + 1) / 2 <= tile
+        uint32_t J = uint32_t((sqrtf(8.0f * float(tile) + 1.0f) - 1.0f) * 0.5f);
+        if (J * (J + 1) / 2 > tile) --J;
+        if ((J + 1) * (J + 2) / 2 <= tile) ++J;
+        const uint32_t I = tile - J * (J + 1) / 2;
+        {
             const uint32_t i = 32 * I + (lane & 31), j = 32 * J + (lane & 31);
             // rows past d read row d - 1: their products land in tile rows
             // and columns that the mask below leaves out
@@ -313,6 +336,7 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
                 acc += s;
             }
         }
+        tile = __shfl(tn, 0);
     }
 }
 
@@ -401,9 +425,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     uint32_t* fb = reinterpret_cast<uint32_t*>(p);
     p += size_t(FW) * 4;
     uint16_t* hv = reinterpret_cast<uint16_t*>(p);
-    __shared__ uint32_t sTask, sTmp[16], sSpan;
-    const int wave = threadIdx.x >> 6;
-    const int nwaves = blockDim.x >> 6;
+    __shared__ uint32_t sTask, sTmp[16], sSpan, sTile;
     const RootSet rs{hk, hv, H - 1, hshift, fb, fshift};
     unsigned long long acc = 0;
     uint32_t qcur = 0xFFFFFFFFu;
@@ -411,7 +433,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     const uint32_t* col = nullptr;
     TaskChunk chunk(ntasks, gridDim.x, 8);
     while (true) {
-        if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
+        if (threadIdx.x == 0) {
+            sTask = thread_next_task(chunk, cursor);
+            sTile = 0;
+        }
         __syncthreads();
         const uint32_t t = sTask;
         if (t >= ntasks) break;
@@ -433,8 +458,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
         __syncthreads();
         build_rows_block<K, NG>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc);
-        if (K == 4) count_rows_mfma4(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
-        else if (K == 5) count_rows<K>(rows, d, d, uint32_t(wave), uint32_t(nwaves), acc);
+        if (K == 4) count_rows_mfma4(rows, d, d, &sTile, acc);
+        else if (K == 5) count_rows<K>(rows, d, d, 0, 0, acc, &sTile);
         __syncthreads();
     }
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
