@@ -217,7 +217,7 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
         if (!ctr) return i + stride;
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(ctr, 1u);
-        return __shfl(t, 0);
+        return wave_bcast(t);
     };
     for (uint32_t i = ctr ? next(0) : first; i + 2 < d; i = next(i)) {
         const unsigned long long* ri = rows + i;
@@ -290,7 +290,7 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
     const uint32_t ntiles = nb * (nb + 1) / 2;
     uint32_t tile = 0;
     if (lane == 0) tile = atomicAdd(ctr, 1u);
-    tile = __shfl(tile, 0);
+    tile = wave_bcast(tile);
     while (tile < ntiles) {
         uint32_t tn = 0;
         if (lane == 0) tn = atomicAdd(ctr, 1u);
@@ -336,7 +336,7 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
                 acc += s;
             }
         }
-        tile = __shfl(tn, 0);
+        tile = wave_bcast(tn);
     }
 }
 
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
             sTile = 0;
         }
         __syncthreads();
-        const uint32_t t = sTask;
+        const uint32_t t = wave_bcast(sTask);
         if (t >= ntasks) break;
         const uint32_t q = task_q ? task_q[t] : 0u;
         if (q != qcur) {
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         }
         const uint32_t u = task_v[t];
         const uint64_t ub = ldg(rowptr, u);
-        const uint32_t d = uint32_t(ldg(rowptr, u + 1) - ub);
+        const uint32_t d = wave_bcast(uint32_t(ldg(rowptr, u + 1) - ub));
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) L[i] = ldg(col, ub + i);
         root_clear(rs, H, FW, threadIdx.x, blockDim.x);
         if (K >= 4)

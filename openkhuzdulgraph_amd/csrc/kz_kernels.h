@@ -87,13 +87,23 @@ struct TaskChunk {
     }
 };
 
+// Lane 0's value as a wave-uniform value (an SGPR: the compiler then keeps
+// what follows from it scalar; __shfl(v, 0) reads as per-lane and turns the
+// loops it bounds into divergent ones).  Call with every lane active.
+__device__ __forceinline__ uint32_t wave_bcast(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// Lane l's value (l wave-uniform) as a wave-uniform value.
+__device__ __forceinline__ uint32_t wave_lane(uint32_t v, uint32_t l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ uint64_t wave_lane(uint64_t v, uint32_t l) {
+    return (uint64_t(wave_lane(uint32_t(v >> 32), l)) << 32) | wave_lane(uint32_t(v), l);
+}
+
 // Next task of a wave (call with every lane; the result is wave-uniform,
 // >= ntasks when none is left).
 __device__ __forceinline__ uint32_t wave_next_task(TaskChunk& c, uint32_t* cursor) {
     if (c.next == c.end) {
         uint32_t v = 0;
         if ((threadIdx.x & 63) == 0) v = atomicAdd(cursor, c.size);
-        c.take(__shfl(v, 0));
+        c.take(wave_bcast(v));
     }
     return c.next++;
 }

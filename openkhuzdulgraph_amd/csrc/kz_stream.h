@@ -322,11 +322,14 @@ __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigne
                                               uint32_t lmax, Hit&& hit, uint32_t* ctr = nullptr) {
     constexpr uint32_t SPAN = 64 * NG;
     const uint32_t lane = tid & 63;
+    // the same in every lane: kept scalar
+    nr = wave_bcast(nr);
+    G = wave_bcast(G);
     if (ctr) {
         const uint32_t nspan = (G + SPAN - 1) / SPAN;
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(ctr, 1u);
-        c = __shfl(c, 0);
+        c = wave_bcast(c);
         while (c < nspan) {
             uint32_t cn = 0;
             if (lane == 0) cn = atomicAdd(ctr, 1u);
@@ -334,7 +337,7 @@ __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigne
             const uint32_t rlo = wave_search(gp, 0, nr, g0);
             const uint32_t rhi = wave_search(gp, rlo, nr, g1) + 1;
             stream_span<NG, kIndex>(col, desc, gp, g0, g1, rlo, rhi, rs, lmax, hit);
-            c = __shfl(cn, 0);
+            c = wave_bcast(cn);
         }
         return;
     }
@@ -434,8 +437,8 @@ __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigne
         while (hm) {
             const int hl = __builtin_ctzll(hm);
             hm &= hm - 1;
-            const uint32_t rr = __shfl(r, hl), nn = __shfl(n, hl), cc = __shfl(cnt, hl);
-            const uint64_t bb = __shfl(b, hl);
+            const uint32_t rr = wave_lane(r, uint32_t(hl)), nn = wave_lane(n, uint32_t(hl)), cc = wave_lane(cnt, uint32_t(hl));
+            const uint64_t bb = wave_lane(b, uint32_t(hl));
             // sample s = a[floor(s * nn / 64)]; nn > 64 (a sampled row has more
             // than kSkewFactor * kSampledRow entries), so the 64 parts are non-empty
             const uint32_t smp = ldg(col, bb + ((uint64_t(lane) * nn) >> 6));
