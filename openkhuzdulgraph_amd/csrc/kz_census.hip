@@ -161,7 +161,7 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
     while (true) {
         if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
         __syncthreads();
-        const uint32_t t = sTask;
+        const uint32_t t = wave_bcast(sTask);
         if (t >= ntasks) break;
         const uint32_t u = tasks[t];
         const uint64_t ub = ldg(frowptr, u);
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void census_c4_kernel(const uint64_t* __restri
     while (true) {
         if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
         __syncthreads();
-        const uint32_t t = sTask;
+        const uint32_t t = wave_bcast(sTask);
         if (t >= n) break;
         const uint32_t u = n - 1 - t;
         const uint64_t ub = ldg(rowptr, u);

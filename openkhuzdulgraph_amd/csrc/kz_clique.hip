@@ -224,6 +224,7 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
         const uint32_t k_lo = (i + 1) >> 6;
         uint32_t total = 0;
         for (uint32_t k = k_lo; k < wd; ++k) total += __popcll(ri[size_t(k) * ws]);
+        total = wave_bcast(total);  // row i is the same in every lane
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t r = base + lane;
             if (r >= total) break;

@@ -39,7 +39,7 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t* ex
         if (lane >= uint32_t(o)) x += y;
     }
     *excl = x - v;
-    return __shfl(x, 63);
+    return wave_lane(x, 63u);
 }
 
 // Block-wide exclusive scan in place over a[0..n) (n <= 64 * blockDim);
@@ -68,7 +68,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t* a, uint32_t n
         run += t;
     }
     __syncthreads();
-    return total;
+    return wave_bcast(total);
 }
 
 // Four consecutive list entries, read with one 16-byte load.
@@ -385,6 +385,12 @@ __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigne
                                               uint32_t d, uint32_t r0, uint32_t nr, uint32_t wave, uint32_t nw,
                                               Hit&& hit) {
     const uint32_t lane = threadIdx.x & 63;
+    // the same in every lane: kept scalar
+    d = wave_bcast(d);
+    r0 = wave_bcast(r0);
+    nr = wave_bcast(nr);
+    wave = wave_bcast(wave);
+    nw = wave_bcast(nw);
     for (uint32_t rbase = wave; rbase < nr; rbase += 64 * nw) {
         const uint32_t r = rbase + lane * nw;
         uint32_t cnt = 0, n = 0;
