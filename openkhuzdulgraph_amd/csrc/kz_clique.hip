@@ -70,10 +70,11 @@ constexpr uint32_t kWaveTaskChunk = 32;
 // One task per wave, |N+(u)| <= 64.  LDS per wave: L[64], 128-slot hash,
 // 64 bitmap rows (one u64 each) and the row descriptors.  Tasks are small,
 // so the chain cursor -> root -> list bounds of the NEXT task is issued
-// while the current one streams and counts.  Register budget: 5 waves per
-// SIMD (at most 96 VGPRs, no spills).
+// while the current one streams and counts.  Register budget: 6 waves per
+// SIMD (at most 80 VGPRs; the kernel is latency-bound, and 5 waves with 96
+// VGPRs measured 32.84 against 32.70 ms on RMAT-22 4-cliques, 7 32.91).
 template <int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void clique_wave_kernel(const QueryRef* __restrict__ queries,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void clique_wave_kernel(const QueryRef* __restrict__ queries,
                                                           const uint32_t* __restrict__ task_v,
                                                           const uint32_t* __restrict__ task_q, uint32_t ntasks,
                                                           uint32_t* __restrict__ cursor,
