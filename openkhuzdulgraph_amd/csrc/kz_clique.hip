@@ -400,7 +400,9 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
 // loses more to fewer waves than the stream gains from more registers.
 // Buckets whose LDS leaves at most 4 waves per SIMD (4-cliques on the
 // 512-1024 bucket) use WPE = 4 and stream NG = 3 groups per lane and span
-// with the registers that frees (RMAT-22 4-clique count 34.97 -> 34.76 ms).
+// with the registers that frees (RMAT-22 4-clique count 34.97 -> 34.76 ms);
+// those whose LDS leaves 7 or more (the 64-128 bucket, latency-bound) use
+// WPE = 7 (72 VGPRs; 32.70 -> 32.43 ms, 8 waves no better).
 template <int K, int NG = kGroups, int WPE = 6>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) void clique_block_kernel(const QueryRef* __restrict__ queries,
                                                            const uint32_t* __restrict__ task_v,
@@ -665,6 +667,7 @@ static int set_lds_attrs() {
                          reinterpret_cast<const void*>(&clique_block_kernel<4>),
                          reinterpret_cast<const void*>(&clique_block_kernel<5>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 3, 4>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4, 2, 7>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<3>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<4>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<5>)};
@@ -714,6 +717,11 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         uint32_t FW, fshift;
         filter_geometry(Dmax, &FW, &fshift);
         if constexpr (K == 4) {
+            if (per_cu * block / 64 >= 28) {  // LDS leaves 7 waves per SIMD or more
+                hipLaunchKernelGGL((clique_block_kernel<4, 2, 7>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
+                                   cur, x.d_result, Dmax, W, H, hshift, FW, fshift);
+                return KZ_OK;
+            }
             if (per_cu * block / 64 <= 16) {  // at most 4 waves per SIMD
                 hipLaunchKernelGGL((clique_block_kernel<4, 3, 4>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
                                    cur, x.d_result, Dmax, W, H, hshift, FW, fshift);
