@@ -193,8 +193,9 @@ __device__ __forceinline__ uint32_t wave_search(const uint32_t* p, uint32_t lo, 
     const uint32_t lane = threadIdx.x & 63;
     while (hi - lo > 1) {
         const uint32_t step = (hi - lo + 63) >> 6;
-        const uint32_t i = lo + lane * step;
-        const unsigned long long b = __ballot(i < hi && p[i] <= f);
+        const uint32_t i = lo + __umul24(lane, step);
+        // lanes past hi read p[hi - 1] (no branch around the read)
+        const unsigned long long b = __ballot((i < hi) & (p[i < hi ? i : hi - 1] <= f));
         lo += (uint32_t(__popcll(b)) - 1) * step;
         hi = hi < lo + step ? hi : lo + step;
     }
