@@ -447,8 +447,11 @@ __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigne
             const uint32_t rr = wave_lane(r, uint32_t(hl)), nn = wave_lane(n, uint32_t(hl)), cc = wave_lane(cnt, uint32_t(hl));
             const uint64_t bb = wave_lane(b, uint32_t(hl));
             // sample s = a[floor(s * nn / 64)]; nn > 64 (a sampled row has more
-            // than kSkewFactor * kSampledRow entries), so the 64 parts are non-empty
-            const uint32_t smp = ldg(col, bb + ((uint64_t(lane) * nn) >> 6));
+            // than kSkewFactor * kSampledRow entries), so the 64 parts are
+            // non-empty.  The row's list a is a wave-uniform base (loads take a
+            // 32-bit offset from it); nn < 2^21, so s * nn fits a 24-bit multiply.
+            const uint32_t* a = col + bb;
+            const uint32_t smp = ldg(a, __umul24(lane, nn) >> 6);
             for (uint32_t cb = 0; cb < cc; cb += 64) {
                 const uint32_t c = cb + lane;
                 const bool live = c < cc;
@@ -461,13 +464,13 @@ __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigne
                     const uint32_t e = __shfl(smp, int(sp + step));
                     if (e <= x) sp += step;
                 }
-                uint32_t at = uint32_t((uint64_t(sp) * nn) >> 6);
-                const uint32_t hi = sp == 63 ? nn : uint32_t((uint64_t(sp + 1) * nn) >> 6);
+                uint32_t at = __umul24(sp, nn) >> 6;
+                const uint32_t hi = sp == 63 ? nn : __umul24(sp + 1, nn) >> 6;
                 uint32_t len = hi - at;
                 uint32_t v = __shfl(smp, int(sp));
                 while (__ballot(len > 1)) {
                     const uint32_t half = len >> 1;
-                    const uint32_t t = ldg(col, bb + at + half);
+                    const uint32_t t = ldg(a, at + half);
                     if (half && t <= x) {
                         at += half;
                         v = t;
