@@ -323,17 +323,28 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
                                                               bits_to_bytes(uint32_t(wj >> (32 + hs)) & 0xFFFFu), c, 0,
                                                               0, 0);
             }
-            // mask: A[i_r][j] for this lane's 16 rows i_r and its column j
+            // mask: A[i_r][j] for this lane's 16 rows i_r and its column j,
+            // read as the 32-bit half of each row's word that holds bit j
+            // (entries of A A^T are at most d < 2^24: 24-bit multiply-adds)
             if (j < d) {
-                const unsigned long long* col_word = rows + size_t(j >> 6) * ws;
-                const uint32_t bit = j & 63;
+                const uint32_t* cw =
+                    reinterpret_cast<const uint32_t*>(rows + size_t(j >> 6) * ws) + ((j >> 5) & 1u);
+                const uint32_t b5 = j & 31u;
                 const uint32_t ib = 32 * I + (lane >> 5) * 4;
                 uint32_t s = 0;
+                if (32 * I + 32 <= d) {  // every row of the tile exists (wave-uniform)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
-                    const uint32_t a = ir < d ? uint32_t(col_word[ir] >> bit) & 1u : 0u;
-                    s += a * uint32_t(c[r]);
+                    for (int r = 0; r < 16; ++r) {
+                        const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
+                        s = __umul24(__builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u), uint32_t(c[r])) + s;
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
+                        const uint32_t a = ir < d ? __builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u) : 0u;
+                        s = __umul24(a, uint32_t(c[r])) + s;
+                    }
                 }
                 acc += s;
             }
