@@ -271,12 +271,14 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
 typedef int kz_v4i __attribute__((ext_vector_type(4)));
 typedef int kz_v16i __attribute__((ext_vector_type(16)));
 
-// 16 bits as 16 bytes of 0 or 1 (bit t in byte t): a 24-bit multiply
-// spreads each nibble's bits 7 apart, the mask keeps one per byte.
-__device__ __forceinline__ kz_v4i bits_to_bytes(uint32_t f) {
+// Bits off .. off + 15 of w as 16 bytes of 0 or 1 (bit off + t in byte t):
+// each nibble is a bit-field extract, a 24-bit multiply spreads its bits 7
+// apart and the mask keeps one per byte.
+__device__ __forceinline__ kz_v4i bits_to_bytes(uint32_t w, uint32_t off) {
     kz_v4i r;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) r[q] = int(__umul24((f >> (4 * q)) & 0xFu, 0x00204081u) & 0x01010101u);
+    for (int q = 0; q < 4; ++q)
+        r[q] = int(__umul24(__builtin_amdgcn_ubfe(w, off + 4u * uint32_t(q), 4u), 0x00204081u) & 0x01010101u);
     return r;
 }
 
@@ -307,22 +309,42 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
             // and columns that the mask below leaves out
             const uint32_t ic = i < d ? i : d - 1, jc = j < d ? j : d - 1;
             const uint32_t hs = 16 * (lane >> 5);
-            // one 64-bit word = two 32-column blocks, read once
+            // one 64-bit word = two 32-column blocks, read once, the next
+            // word's reads issued before this one's products; word J / 2 may
+            // start at its high block (J odd), the last word may end after
+            // its low block (nb odd)
             kz_v16i c;
 #pragma unroll
             for (int r = 0; r < 16; ++r) c[r] = 0;
             const uint32_t wend = (nb + 1) >> 1;
-            for (uint32_t kw = J >> 1; kw < wend; ++kw) {
-                const unsigned long long wi = rows[size_t(kw) * ws + ic];
-                const unsigned long long wj = rows[size_t(kw) * ws + jc];
-                if (2 * kw >= J)
-                    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi >> hs) & 0xFFFFu),
-                                                              bits_to_bytes(uint32_t(wj >> hs) & 0xFFFFu), c, 0, 0, 0);
-                if (2 * kw + 1 < nb)
-                    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi >> (32 + hs)) & 0xFFFFu),
-                                                              bits_to_bytes(uint32_t(wj >> (32 + hs)) & 0xFFFFu), c, 0,
-                                                              0, 0);
+            uint32_t kw = J >> 1;
+            unsigned long long wi = rows[size_t(kw) * ws + ic], wj = rows[size_t(kw) * ws + jc];
+            if (J & 1u) {
+                const uint32_t hi_i = uint32_t(wi >> 32), hi_j = uint32_t(wj >> 32);
+                ++kw;
+                if (kw < wend) {
+                    wi = rows[size_t(kw) * ws + ic];
+                    wj = rows[size_t(kw) * ws + jc];
+                }
+                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(hi_i, hs), bits_to_bytes(hi_j, hs), c, 0, 0,
+                                                          0);
             }
+            while (2 * kw + 1 < nb) {
+                const uint32_t lo_i = uint32_t(wi), hi_i = uint32_t(wi >> 32);
+                const uint32_t lo_j = uint32_t(wj), hi_j = uint32_t(wj >> 32);
+                ++kw;
+                if (kw < wend) {
+                    wi = rows[size_t(kw) * ws + ic];
+                    wj = rows[size_t(kw) * ws + jc];
+                }
+                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(lo_i, hs), bits_to_bytes(lo_j, hs), c, 0, 0,
+                                                          0);
+                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(hi_i, hs), bits_to_bytes(hi_j, hs), c, 0, 0,
+                                                          0);
+            }
+            if (2 * kw < nb)
+                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi), hs),
+                                                          bits_to_bytes(uint32_t(wj), hs), c, 0, 0, 0);
             // mask: A[i_r][j] for this lane's 16 rows i_r and its column j,
             // read as the 32-bit half of each row's word that holds bit j
             // (entries of A A^T are at most d < 2^24: 24-bit multiply-adds)
