@@ -439,45 +439,88 @@ __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigne
             }
             if (live && v == x) hit(rr, jj, bb + at);
         }
-        // heavy rows: sample, then search one 1/64 of the list
+        // heavy rows: sample, then search one 1/64 of the list.  Sample s
+        // = a[floor(s * n / 64)] (a the row's list); n > 64 (a sampled row
+        // has more than kSkewFactor * kSampledRow entries), so the 64 parts
+        // are non-empty; n < 2^21, so s * n fits a 24-bit multiply.  The
+        // heavy rows' candidates go 64 per iteration as one sequence: an
+        // iteration may take the end of row A in lanes [0, split) and the
+        // start of the next row B in the others, each row's samples in a
+        // register of its own, so a row's last partial iteration does not
+        // leave lanes idle.
         unsigned long long hm = __ballot(sampled);
-        while (hm) {
-            const int hl = __builtin_ctzll(hm);
+        auto next_row = [&](uint32_t& rr, uint32_t& nn, uint32_t& cc, uint64_t& bb, uint32_t& smp) {
+            const uint32_t hl = uint32_t(__builtin_ctzll(hm));
             hm &= hm - 1;
-            const uint32_t rr = wave_lane(r, uint32_t(hl)), nn = wave_lane(n, uint32_t(hl)), cc = wave_lane(cnt, uint32_t(hl));
-            const uint64_t bb = wave_lane(b, uint32_t(hl));
-            // sample s = a[floor(s * nn / 64)]; nn > 64 (a sampled row has more
-            // than kSkewFactor * kSampledRow entries), so the 64 parts are
-            // non-empty.  The row's list a is a wave-uniform base (loads take a
-            // 32-bit offset from it); nn < 2^21, so s * nn fits a 24-bit multiply.
-            const uint32_t* a = col + bb;
-            const uint32_t smp = ldg(a, __umul24(lane, nn) >> 6);
-            for (uint32_t cb = 0; cb < cc; cb += 64) {
-                const uint32_t c = cb + lane;
-                const bool live = c < cc;
-                const uint32_t jj = r0 + rr + 1 + c;
-                const uint32_t x = live ? L[jj] : 0u;
-                // the part: the largest s with sample s <= x (s = 0 if none)
-                uint32_t sp = 0;
+            rr = wave_lane(r, hl);
+            nn = wave_lane(n, hl);
+            cc = wave_lane(cnt, hl);
+            bb = wave_lane(b, hl);
+            smp = ldg(col + bb, __umul24(lane, nn) >> 6);
+        };
+        uint32_t rA = 0, nA = 0, cA = 0, sA = 0, oA = 0;  // oA: row A's candidates done
+        uint64_t bA = 0;
+        bool more = hm != 0;
+        if (more) next_row(rA, nA, cA, bA, sA);
+        while (more) {
+            const uint32_t remA = cA - oA;
+            const bool mixed = remA < 64 && hm != 0;
+            uint32_t rB = 0, nB = 0, cB = 0, sB = 0;
+            uint64_t bB = 0;
+            if (mixed) next_row(rB, nB, cB, bB, sB);
+            const uint32_t split = remA < 64 ? remA : 64u;
+            const bool inB = lane >= split;
+            const uint32_t c = inB ? lane - split : oA + lane;
+            const bool live = inB ? (mixed && c < cB) : true;
+            const uint32_t rr = inB ? rB : rA, nn = inB ? nB : nA;
+            const uint64_t bb = inB ? bB : bA;
+            const uint32_t jj = r0 + rr + 1 + c;
+            const uint32_t x = live ? L[jj] : 0u;
+            // the part: the largest s with sample s <= x (s = 0 if none)
+            uint32_t sp = 0;
 #pragma unroll
-                for (uint32_t step = 32; step; step >>= 1) {
-                    const uint32_t e = __shfl(smp, int(sp + step));
-                    if (e <= x) sp += step;
+            for (uint32_t step = 32; step; step >>= 1) {
+                uint32_t e = __shfl(sA, int(sp + step));
+                if (mixed) {
+                    const uint32_t eB = __shfl(sB, int(sp + step));
+                    e = inB ? eB : e;
                 }
-                uint32_t at = __umul24(sp, nn) >> 6;
-                const uint32_t hi = sp == 63 ? nn : __umul24(sp + 1, nn) >> 6;
-                uint32_t len = hi - at;
-                uint32_t v = __shfl(smp, int(sp));
-                while (__ballot(len > 1)) {
-                    const uint32_t half = len >> 1;
-                    const uint32_t t = ldg(a, at + half);
-                    if (half && t <= x) {
-                        at += half;
-                        v = t;
-                    }
-                    len -= half;
+                if (e <= x) sp += step;
+            }
+            uint32_t at = __umul24(sp, nn) >> 6;
+            const uint32_t hi = sp == 63 ? nn : __umul24(sp + 1, nn) >> 6;
+            uint32_t len = live ? hi - at : 0u;
+            uint32_t v = __shfl(sA, int(sp));
+            if (mixed) {
+                const uint32_t vB = __shfl(sB, int(sp));
+                v = inB ? vB : v;
+            }
+            const uint32_t* a = col + bb;
+            while (__ballot(len > 1)) {
+                const uint32_t half = len >> 1;
+                const uint32_t t = ldg(a, at + half);
+                if (half && t <= x) {
+                    at += half;
+                    v = t;
                 }
-                if (live && v == x) hit(rr, jj, bb + at);
+                len -= half;
+            }
+            if (live && v == x) hit(rr, jj, bb + at);
+            // row A of the next iteration
+            if (remA > 64) {
+                oA += 64;
+            } else if (mixed && cB > 64 - split) {
+                rA = rB;
+                nA = nB;
+                cA = cB;
+                bA = bB;
+                sA = sB;
+                oA = 64 - split;
+            } else if (hm) {
+                next_row(rA, nA, cA, bA, sA);
+                oA = 0;
+            } else {
+                more = false;
             }
         }
     }
