@@ -378,8 +378,7 @@ __device__ __forceinline__ uint32_t find_global(const uint32_t* __restrict__ a, 
 //     only that part (64 lines per row, then about 1 - 2 per candidate).
 // No workgroup barrier, so it overlaps other waves' streaming.  hit(r, j, p):
 // candidate j found at position p of col.  Call with every lane of the wave.
-constexpr uint32_t kSampledRow = 16;
-static_assert(kSkewFactor * kSampledRow >= 64, "a sampled row must have more than 64 entries");
+constexpr uint32_t kSampledRow = 12;  // RMAT-22 4-cliques: 4 / 8 / 12 / 16 / 24: 31.4 / 30.9 / 30.9 / 31.0 / 31.3 ms
 
 template <class Hit>
 __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigned long long* desc, const uint32_t* L,
@@ -440,9 +439,10 @@ __device__ __forceinline__ void search_skewed(const uint32_t* col, const unsigne
             if (live && v == x) hit(rr, jj, bb + at);
         }
         // heavy rows: sample, then search one 1/64 of the list.  Sample s
-        // = a[floor(s * n / 64)] (a the row's list); n > 64 (a sampled row
-        // has more than kSkewFactor * kSampledRow entries), so the 64 parts
-        // are non-empty; n < 2^21, so s * n fits a 24-bit multiply.  The
+        // = a[floor(s * n / 64)] (a the row's list; n < 2^21, so s * n fits
+        // a 24-bit multiply).  With n < 64 samples repeat, but the largest s
+        // with sample s <= x then has a non-empty part (its successor's
+        // sample differs), and x below sample 0 is absent.  The
         // heavy rows' candidates go 64 per iteration as one sequence: an
         // iteration may take the end of row A in lanes [0, split) and the
         // start of the next row B in the others, each row's samples in a
