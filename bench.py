@@ -439,7 +439,7 @@ def main() -> int:
         print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)",
+        print("bench.py measures one MI355X: the library has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         return 2
 

@@ -9,7 +9,7 @@
 // prefixes, followed by candidate loops that either scatter a child
 // (engine->scatter) or add a count (Aggregator::add), filtered by comparisons
 // of the candidate with the matched vertices.  Every one of those touches a
-// symbol or inline accessor this build provides, so it can be observed.
+// symbol or inline accessor the library provides, so it can be observed.
 //
 // What is observed, for embeddings of size s and ORDER TYPE t (the relative
 // order of the matched vertices, ties included):
