@@ -191,7 +191,7 @@ __global__ __launch_bounds__(T) void census_tri_kernel(const uint64_t* __restric
             atomicAdd(&cnt[j], 1u);
             atomicAdd(&tmid[at], 1u);
         };
-        stream_groups(fcol, rb, rp, nr, G, threadIdx.x, T, rs, L[d - 1], hit, T > 64 ? &sSpan : nullptr);
+        stream_groups(fcol, rb, rp, nr, G, threadIdx.x, T, rs, hit, T > 64 ? &sSpan : nullptr);
         // skewed rows: each wave searches the candidates of its own rows
         search_skewed(fcol, rb, L, d, 0, nr, threadIdx.x >> 6, T / 64, hit);
         __syncthreads();

@@ -146,8 +146,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
             un = task_v[tn];
         }
         wave_lds_sync();
-        const uint32_t lmax = L[d - 1];
-        stream_groups<kGroups, (K >= 4)>(col, rb, rp, d - 1, G, lane, 64, rs, lmax,
+        stream_groups<kGroups, (K >= 4)>(col, rb, rp, d - 1, G, lane, 64, rs,
                       [&](uint32_t r, uint32_t j, uint64_t) {
                           if (K == 3) acc += 1;
                           else atomicOr(&rows[r], 1ull << j);
@@ -410,7 +409,7 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
     __syncthreads();
     const uint32_t G = block_exclusive_scan(rp, nr + 1, tmp);
     stream_groups<NG, (K >= 4)>(
-        col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs, L[d - 1],
+        col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs,
         [&](uint32_t r, uint32_t j, uint64_t) {
             if (K == 3) acc += 1;
             else set_row_bit(rows_out, ws, r, j);
