@@ -434,7 +434,8 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
 // 512-1024 bucket) use WPE = 4 and stream NG = 3 groups per lane and span
 // with the registers that frees (RMAT-22 4-clique count 34.97 -> 34.76 ms);
 // those whose LDS leaves 7 or more (the 64-128 bucket, latency-bound) use
-// WPE = 7 (72 VGPRs; 32.70 -> 32.43 ms, 8 waves no better).
+// WPE = 7 (72 VGPRs; 32.70 -> 32.43 ms, 8 waves no better), and so do
+// 3-cliques, which keep no bitmap (25.81 -> 25.46 ms).
 template <int K, int NG = kGroups, int WPE = 6>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) void clique_block_kernel(const QueryRef* __restrict__ queries,
                                                            const uint32_t* __restrict__ task_v,
@@ -700,6 +701,7 @@ static int set_lds_attrs() {
                          reinterpret_cast<const void*>(&clique_block_kernel<5>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 3, 4>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 2, 7>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<3, 2, 7>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<3>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<4>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<5>)};
@@ -748,6 +750,13 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         const uint32_t grid = std::min<uint32_t>(nt, uint32_t(R.num_cus) * per_cu);
         uint32_t FW, fshift;
         filter_geometry(Dmax, &FW, &fshift);
+        if constexpr (K == 3) {  // no bitmap: the LDS leaves 7 waves per SIMD or more
+            if (per_cu * block / 64 >= 28) {
+                hipLaunchKernelGGL((clique_block_kernel<3, 2, 7>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
+                                   cur, x.d_result, Dmax, W, H, hshift, FW, fshift);
+                return KZ_OK;
+            }
+        }
         if constexpr (K == 4) {
             if (per_cu * block / 64 >= 28) {  // LDS leaves 7 waves per SIMD or more
                 hipLaunchKernelGGL((clique_block_kernel<4, 2, 7>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
