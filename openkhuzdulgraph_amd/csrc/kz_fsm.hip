@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void label_sort_kernel(const uint64_t* __restr
             unsigned long long todo = __ballot(live);
             while (todo) {
                 const uint32_t lead = uint32_t(__builtin_ctzll(todo));
-                const uint32_t ll = __shfl(l, int(lead));
+                const uint32_t ll = wave_lane(l, lead);
                 const unsigned long long m = __ballot(l == ll);
                 const uint32_t at = base[ll];
                 if (l == ll) {
@@ -194,8 +194,8 @@ __device__ __forceinline__ void fold_exclusions(bool ok, uint32_t x1, uint32_t x
     if (!okm) return;
     if (!any) {
         const int f = __builtin_ctzll(okm);
-        i1 = __shfl(x1, f);
-        i2 = __shfl(x2, f);
+        i1 = wave_lane(x1, uint32_t(f));
+        i2 = wave_lane(x2, uint32_t(f));
         any = true;
     }
     if (i1 != kNone && __ballot(ok && x1 != i1 && x2 != i1)) i1 = kNone;
