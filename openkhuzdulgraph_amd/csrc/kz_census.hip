@@ -229,14 +229,20 @@ __global__ __launch_bounds__(256) void census_sums_kernel(const uint64_t* __rest
     }
 }
 
-// Pass 3: 4-cycles.  Tasks are all vertices, highest rank first; each
-// workgroup owns n counters and an n-entry touched list in HBM.
+// Pass 3: 4-cycles.  Tasks are all vertices, highest rank first.  A root
+// whose wedges (bounded by the degrees of its neighbours ranked below it)
+// fit an LDS hash of kC4Slots counters counts there; the others use the
+// workgroup's n counters and n-entry touched list in HBM (one returning
+// atomic per wedge, the cost of this pass before the LDS counters).
+constexpr uint32_t kC4Slots = 4096;
+
 __global__ __launch_bounds__(256) void census_c4_kernel(const uint64_t* __restrict__ rowptr,
                                                         const uint32_t* __restrict__ col, uint32_t n,
                                                         uint32_t* __restrict__ cursor, uint32_t* __restrict__ cnt_slab,
                                                         uint32_t* __restrict__ touched_slab,
                                                         unsigned long long* __restrict__ sums) {
-    __shared__ uint32_t sTask, sTouched;
+    __shared__ uint32_t sTask, sTouched, sBound;
+    __shared__ uint32_t sKey[kC4Slots], sCnt[kC4Slots];
     uint32_t* cnt = cnt_slab + size_t(blockIdx.x) * n;
     uint32_t* touched = touched_slab + size_t(blockIdx.x) * n;
     const uint32_t lane = threadIdx.x & 63;
@@ -246,7 +252,10 @@ __global__ __launch_bounds__(256) void census_c4_kernel(const uint64_t* __restri
     if (threadIdx.x == 0) sTouched = 0;
     TaskChunk chunk(n, gridDim.x, 8);
     while (true) {
-        if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
+        if (threadIdx.x == 0) {
+            sTask = thread_next_task(chunk, cursor);
+            sBound = 0;
+        }
         __syncthreads();
         const uint32_t t = wave_bcast(sTask);
         if (t >= n) break;
@@ -254,6 +263,62 @@ __global__ __launch_bounds__(256) void census_c4_kernel(const uint64_t* __restri
         const uint64_t ub = ldg(rowptr, u);
         const uint32_t du = uint32_t(ldg(rowptr, u + 1) - ub);
         const uint32_t pre = lower_bound_global(col + ub, du, u);  // neighbours ranked below u
+        // wedge bound: the degrees of the neighbours below u
+        // (64-bit sums, each wave's clamped to 2^16 before the LDS add, so
+        // that no sum wraps and a root above the LDS size never takes it)
+        unsigned long long part = 0;
+        for (uint32_t k = threadIdx.x; k < pre; k += blockDim.x) {
+            const uint32_t v = ldg(col, ub + k);
+            part += ldg(rowptr, v + 1) - ldg(rowptr, v);
+        }
+        part = wave_sum_u64(part);
+        if (lane == 0 && part) atomicAdd(&sBound, uint32_t(part < 65536ull ? part : 65536ull));
+        __syncthreads();
+        const uint32_t bound = wave_bcast(sBound);
+        if (bound <= kC4Slots / 2) {
+            // LDS counters: an open-addressing hash of at least twice the
+            // wedges, so every probe ends
+            uint32_t H = 64;
+            while (H < 2 * bound) H <<= 1;
+            uint32_t bits = 0;
+            while ((1u << bits) < H) ++bits;
+            const uint32_t hshift = 32 - bits;
+            for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) {
+                sKey[i] = kEmpty;
+                sCnt[i] = 0;
+            }
+            __syncthreads();
+            for (uint32_t k = wave; k < pre; k += nw) {
+                const uint32_t v = ldg(col, ub + k);
+                const uint64_t vb = ldg(rowptr, v);
+                const uint32_t dv = uint32_t(ldg(rowptr, v + 1) - vb);
+                for (uint32_t base = 0; base < dv; base += 64) {
+                    const uint32_t i = base + lane;
+                    const uint32_t w = i < dv ? ldg(col, vb + i) : 0xFFFFFFFFu;
+                    if (w < u) {
+                        uint32_t h = hash_slot(w, hshift);
+                        while (true) {
+                            const uint32_t key = sKey[h];
+                            if (key == w) break;
+                            if (key == kEmpty) {
+                                const uint32_t old = atomicCAS(&sKey[h], kEmpty, w);
+                                if (old == kEmpty || old == w) break;
+                            }
+                            h = (h + 1) & (H - 1);
+                        }
+                        atomicAdd(&sCnt[h], 1u);
+                    }
+                    if (__ballot(w >= u)) break;  // sorted lists: the rest ranks above u
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) {
+                const uint64_t c = sCnt[i];
+                acc += c * (c - 1) / 2;
+            }
+            __syncthreads();
+            continue;
+        }
         for (uint32_t k = wave; k < pre; k += nw) {
             const uint32_t v = ldg(col, ub + k);
             const uint64_t vb = ldg(rowptr, v);
