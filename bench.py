@@ -1,19 +1,22 @@
 """Benchmark: exact 4-clique count on a seeded RMAT graph, one MI355X.
 
-Workload (default): RMAT scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05,
+Workload (default): RMAT scale 24, edge factor 16, a/b/c/d = .57/.19/.19/.05,
 seed 1; symmetrised, deduplicated, self-loops and isolated vertices dropped,
 degree-oriented; uint32 CSR written to and loaded back from a .dgraph file.
 The graph is resident in HBM before timing starts.
 
 A step is one full exact count of the graph's 4-cliques.  The persistent
 form (one hipGraph launch of the degree-bucketed kernels) is the headline;
-the plain level-by-level form is timed beside it.  Throughput `value` is the
-merge-model algorithmic bytes B4 of the count divided by the persistent
-form's seconds (GB/s).  The CPU baseline is the oracle's restatement of the
-kautomine four_clique schedule on a bounded sample of roots, on this host's
-cores.
+the plain level-by-level form is timed beside it.  `value` is the persistent
+form's seconds per count; the count is checked against the committed
+oracle count of the same graph (tests/golden/config_counts.json, matched by
+CSR digest).  The roofline uses the bytes the kernels must stream,
+4 (m + sum over edges (u, v) of d+(v)), over the HIP-event time of the
+launch.  The CPU baseline is the oracle's restatement of the kautomine
+four_clique schedule on a bounded sample of roots, on this host's cores,
+with its time projected to the whole count by merge-model bytes.
 
-    python bench.py [--steps K] [--warmup W] [--scale 22] [--plain-steps 1]
+    python bench.py [--steps K] [--warmup W] [--scale 24] [--plain-steps 1]
 
 Other workloads (each prints its own JSON line; the default is the one above):
     --workload census   induced 3/4-vertex census of a seeded Chung-Lu graph
@@ -78,35 +81,21 @@ def cpu_baseline(g, budget_s: float, threads: int) -> dict:
     dt = time.perf_counter() - t0
     return {
         "value": b["B4"] / dt / 1e9,
-        "unit": "GB/s",
+        "unit": "GB/s (merge-model bytes of the sample / its seconds)",
         "cores": threads,
         "kind": "port",
         "sample": f"every {stride}-th root vertex ({len(roots)} of {n}) of the same graph, "
                   f"{cnt[0]} 4-cliques, {b['B4']} merge-model bytes, {dt:.2f} s",
         "seconds": dt,
+        "sample_bytes": b["B4"],
     }
 
 
 def pool_graph(kz, i: int, gpu: bool = False, orient: bool = False):
-    """Graph i of the batch workload (seed 1000 + i): even i Erdos-Renyi,
-    odd i RMAT (.57/.19/.19/.05); n ~ U[2^10, 2^18], average degree ~ U[4, 64]
-    (RMAT: scale round(log2 n), edge factor round(degree / 2)).  Host CSR, or
-    with gpu=True the same graph built in HBM (a DeviceGraph)."""
-    import math
+    """Graph i of the batch workload (openkhuzdulgraph_amd.workloads)."""
+    from openkhuzdulgraph_amd import workloads as wl
 
-    import numpy as np
-
-    rng = np.random.default_rng(1000 + i)
-    n = int(rng.integers(2**10, 2**18 + 1))
-    deg = float(rng.uniform(4, 64))
-    if gpu:
-        if i % 2 == 0:
-            return kz.gpu_gen_erdos_renyi(n, int(n * deg / 2), 1000 + i, orient)
-        return kz.gpu_gen_rmat(max(10, round(math.log2(n))), max(2, round(deg / 2)), 0.57, 0.19, 0.19, 1000 + i,
-                               orient)
-    g = (kz.gen_erdos_renyi(n, int(n * deg / 2), 1000 + i) if i % 2 == 0 else
-         kz.gen_rmat(max(10, round(math.log2(n))), max(2, round(deg / 2)), 0.57, 0.19, 0.19, 1000 + i))
-    return g.oriented() if orient else g
+    return wl.pool_graph(kz, i, gpu=gpu, orient=orient)
 
 
 def base_line(metric: str, value, unit: str, steps: int, warmup: int, ms: float, higher: bool, config: dict) -> dict:
@@ -117,7 +106,9 @@ def base_line(metric: str, value, unit: str, steps: int, warmup: int, ms: float,
 
 def bench_census(kz, args) -> int:
     t0 = time.perf_counter()
-    g = kz.gen_chung_lu(4850000, 43000000, 2.3, 20000, 1)
+    from openkhuzdulgraph_amd import workloads as wl
+
+    g = wl.chunglu_census(kz)
     log(f"generated {g} in {time.perf_counter() - t0:.1f} s")
     kz.init(0)
     dg = g.to_device()
@@ -400,12 +391,46 @@ def bench_gen(kz, args) -> int:
     return 0
 
 
+def copy_probe() -> int:
+    """Known-size kernels for the HBM byte counters and the measured copy
+    bandwidth: 3 copies of 1 GiB (1 GiB read + 1 GiB written each), then 3
+    read-only sums of the same 1 GiB (nothing written but the sum), then 20
+    timed copies of 4 GiB."""
+    import torch
+
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        a.sum(dtype=torch.int64)
+    torch.cuda.synchronize()
+    del a, b
+    big = 1 << 30
+    x = torch.ones(big, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    print(json.dumps({"probe_bytes_read": n * 4, "probe_bytes_written": n * 4, "probe_sum_bytes_read": n * 4,
+                      "copy_GBps": 2 * 4 * big / (ms / 1e3) / 1e9, "copy_bytes_each": 2 * 4 * big,
+                      "copy_ms": ms}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--k", type=int, default=4)
@@ -414,7 +439,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dgraph", action="store_true", help="skip the .dgraph round trip")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
+                    help="only run known-size copies and read-only sums (calibrates the HBM byte counters) "
+                         "and time a 4 GiB copy")
     ap.add_argument("--workload", choices=["clique", "census", "batch", "serve", "gen", "fsm"], default="clique")
     ap.add_argument("--labels", type=int, default=8, help="fsm: number of vertex labels")
     ap.add_argument("--threshold", type=int, default=20000, help="fsm: MNI support threshold")
@@ -429,15 +455,7 @@ def main() -> int:
     ap.add_argument("--large-share", type=float, default=0.5, help="serve: share of queries on the large graph")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-
-        a = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4}))
-        return 0
+        return copy_probe()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the library has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
@@ -456,26 +474,60 @@ def main() -> int:
     if args.workload == "gen":
         return bench_gen(kz, args)
 
+    return bench_clique(kz, args)
+
+
+def golden_entry(args, digest: str):
+    """The committed exact count of this workload's graph (oracle and, for
+    the small graphs, the reference binaries), when one exists for it."""
+    path = ROOT / "tests" / "golden" / "config_counts.json"
+    if not path.exists() or args.seed != 1:
+        return None
+    for e in json.loads(path.read_text()).values():
+        if e.get("digest") == digest and e.get("oriented"):
+            apps = {3: "kautomine_triangle", 4: "kautomine_four_clique", 5: "kautomine_five_clique"}
+            c = e.get("counts", {}).get(f"oracle:{apps[args.k]}")
+            if c is not None:
+                return c
+    return None
+
+
+def host_cpu() -> dict:
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": usable, "cpu_model": model}
+
+
+def bench_clique(kz, args) -> int:
+    from openkhuzdulgraph_amd import workloads as wl
+
     t0 = time.perf_counter()
     if args.gpu_gen:
         kz.init(0)
-        dg = kz.gpu_gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed, orient=True)
+        dg = wl.rmat(kz, args.scale, args.edge_factor, args.seed, gpu=True)
         log(f"built on the GPU: {dg.num_vertices} vertices, {dg.num_entries} oriented entries in "
             f"{time.perf_counter() - t0:.1f} s")
-        g = dg.download() if not args.no_cpu else None
-        digest = g.digest() if g is not None else 0
+        g = dg.download()
     else:
-        sym = kz.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
-        g = sym.oriented()
-        del sym
+        g = wl.rmat(kz, args.scale, args.edge_factor, args.seed)
         log(f"generated {g} in {time.perf_counter() - t0:.1f} s")
-        digest = g.digest()
+    digest = f"{g.digest():016x}"
     if not args.no_dgraph and not args.gpu_gen:
         with tempfile.TemporaryDirectory() as td:
             path = os.path.join(td, f"rmat{args.scale}.dgraph")
             g.store(path)
             g2 = kz.load_dgraph(path)
-            if g2.digest() != digest:
+            if f"{g2.digest():016x}" != digest:
                 raise SystemExit(".dgraph round trip changed the graph")
             g = g2
         log(".dgraph store/load round trip ok")
@@ -501,7 +553,11 @@ def main() -> int:
             raise SystemExit(f"persistent count changed between steps: {c} vs {count}")
     t_pers = (time.perf_counter() - t0) / args.steps
     count = c
-    log(f"persistent: count={count} {t_pers * 1e3:.2f} ms/step (device {sum(dev_ms) / len(dev_ms):.2f} ms)")
+    dev_avg_ms = sum(dev_ms) / len(dev_ms)
+    log(f"persistent: count={count} {t_pers * 1e3:.2f} ms/step (device {dev_avg_ms:.2f} ms)")
+    golden = golden_entry(args, digest)
+    if golden is not None and golden != count:
+        raise SystemExit(f"count {count} differs from the committed oracle count {golden}")
 
     # plain level-by-level form
     plain = None
@@ -516,40 +572,44 @@ def main() -> int:
         st = dg.last_stats
         t_plain = min(tp)
         plain = {"seconds": t_plain, "device_ms": st["device_ms"], "chunks": st["chunks"],
-                 "launches": st["launches"], "embeddings": st["embeddings"],
-                 "GB/s": work["B4"] / t_plain / 1e9}
+                 "launches": st["launches"], "embeddings": st["embeddings"]}
         log(f"plain: {t_plain:.3f} s")
 
     B4 = work["B4"] if args.k == 4 else None
-    dev_avg_ms = sum(dev_ms) / len(dev_ms)
-    # bytes the vertex-centric kernel must stream: N+(u) once per root and
+    # bytes the vertex-centric kernels must stream: N+(u) once per root and
     # N+(v) once per edge (u, v) (column ids, 4 B each)
     b_vc = 4 * (work["m"] + work["sum_dv"])
     achieved = b_vc / (dev_avg_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "clique_block_kernel/clique_wave_kernel (persistent bucket graph)",
+                "kernel": "clique_block_kernel/clique_wave_kernel (persistent bucket graph, one hipGraph launch)",
                 "bytes_per_launch": b_vc, "avg_launch_ms": dev_avg_ms}
     traffic_file = ROOT / "profiles" / f"rmat{args.scale}_k{args.k}_persistent_traffic.json"
     if traffic_file.exists():
-        roofline["traffic"] = json.loads(traffic_file.read_text()).get("bytes_per_launch")
+        tr = json.loads(traffic_file.read_text())
+        roofline["traffic"] = tr.get("bytes_per_launch")
+        roofline["l2_hit_rate"] = tr.get("l2_hit_rate")
+        roofline["measured_copy_GBps"] = tr.get("copy_GBps")
 
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         cpu = cpu_baseline(g, args.cpu_seconds, threads)
+        if B4:
+            # the sample's merge-model work scaled to the whole count
+            cpu["projected_full_count_seconds"] = cpu["seconds"] * B4 / cpu["sample_bytes"]
+        cpu.update(host_cpu())
         log(f"cpu baseline: {cpu}")
 
-    value = (B4 / t_pers / 1e9) if B4 else None
     line = {
         "metric": METRIC,
-        "value": value,
-        "unit": "GB/s",
+        "value": t_pers,
+        "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": t_pers * 1e3,
-        "higher_is_better": True,
+        "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "uint32",
@@ -559,13 +619,17 @@ def main() -> int:
             "generator": "rmat a/b/c/d=0.57/0.19/0.19/0.05, symmetrised, deduped, no loops/isolated, degree-oriented",
             "scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed, "k": args.k,
             "vertices": dg.num_vertices, "oriented_edges": dg.num_entries, "max_out_degree": dg.max_degree,
-            "csr_digest": f"{digest:016x}",
+            "csr_digest": digest,
         },
         "count": count,
+        "golden_count": golden,
         "seconds": t_pers,
+        "device_ms": dev_avg_ms,
         "counts_per_s": count / t_pers,
         "edges_per_s": dg.num_entries / t_pers,
+        "streamed_GBps": achieved,
         "merge_model_bytes": B4,
+        "merge_model_equivalent_GBps": (B4 / t_pers / 1e9) if B4 else None,
         "plain": plain,
         "speedup_vs_plain": (plain["seconds"] / t_pers) if plain else None,
         "roofline": roofline,

@@ -163,6 +163,13 @@ int kz_gpu_gen_chung_lu(uint32_t n, uint64_t num_draws, double exponent, uint32_
 /* Copies a resident graph back into a host CSR (released with
  * kz_csr_release); e.g. to write it as .dgraph. */
 int kz_graph_download(kz_graph g, kz_csr* out);
+/* Degree orientation of a resident symmetric graph on the GPU: a new
+ * resident graph, the CSR kz_orient_by_degree builds on the host (rank by
+ * (degree asc, id asc), relabel by rank, keep u->v iff rank(v) > rank(u),
+ * lists sorted).  Replaces toolkits/to_oriented_dataset_converter.cc:27-124,
+ * the offline step the clique apps' oriented inputs come from.  The clique
+ * counts of a symmetric graph run on this orientation of it. */
+int kz_graph_orient(kz_graph sym, kz_graph* out);
 /* Largest number of keys one radix-sort call of the GPU generators takes
  * (default 2^30; smaller values exercise the chunked path in tests). */
 int kz_set_sort_chunk(uint64_t items);
@@ -195,8 +202,10 @@ typedef struct kz_run_stats {
 
 /* Exact k-clique count (k = 3..5).  Equals the count printed by
  * kautomine/{triangle,four_clique,five_clique}.cc and the kgraphpi apps of
- * the same names on the same .dgraph.  The persistent form needs every
- * forward out-degree <= 8192 (true after degree orientation). */
+ * the same names on the same .dgraph, symmetric or degree-oriented.  The
+ * persistent form runs on the degree orientation of a symmetric graph
+ * (kz_graph_orient, built once and kept with the graph); a graph whose
+ * forward out-degree still exceeds 8192 is counted by the plain form. */
 int kz_count_cliques(kz_graph g, int k, int method, uint64_t* count, kz_run_stats* stats);
 
 /* Degree buckets of the persistent form: root tasks with |N+(u)| <= bound[0]

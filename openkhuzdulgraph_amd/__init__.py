@@ -361,6 +361,13 @@ class DeviceGraph:
         check(lib().kz_graph_download(self._h, C.byref(out)))
         return CSRGraph(out)
 
+    def oriented(self) -> "DeviceGraph":
+        """Degree orientation of this symmetric graph built on the GPU
+        (kz_graph_orient): the same CSR as download().oriented()."""
+        h = C.c_void_p()
+        check(lib().kz_graph_orient(self._h, C.byref(h)))
+        return DeviceGraph(h)
+
     def count_cliques(self, k: int, method: str = "persistent") -> int:
         out = C.c_uint64()
         st = _ffi.kz_run_stats()

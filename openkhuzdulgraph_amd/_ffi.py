@@ -220,6 +220,7 @@ _SIGS = {
     "kz_gpu_gen_chung_lu": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_uint32, C.c_uint64, C.c_int,
                                       C.POINTER(C.c_void_p)]),
     "kz_graph_download": (C.c_int, [C.c_void_p, C.POINTER(kz_csr)]),
+    "kz_graph_orient": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "kz_graph_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_int)]),
     "kz_set_sort_chunk": (C.c_int, [C.c_uint64]),

@@ -49,30 +49,34 @@ int ensure_forward(DeviceGraph& g) {
         g.fwd_rowptr = g.rowptr;
         g.fwd_col = g.col;
         g.h_fdeg = g.h_deg;
+        g.fwd_bytes = 0;
+        g.fwd_max = g.max_degree;
         g.fwd_ready = true;
         return KZ_OK;
     }
-    hipStream_t s = rt().stream;
-    uint32_t *skip = nullptr, *fdeg = nullptr;
-    KZ_HIP(hipMalloc(&skip, sizeof(uint32_t) * std::max<uint32_t>(1, g.n)));
-    KZ_HIP(hipMalloc(&fdeg, sizeof(uint32_t) * std::max<uint32_t>(1, g.n)));
-    if (g.n) hipLaunchKernelGGL(fwd_start_kernel, dim3((g.n + 255) / 256), dim3(256), 0, s, g.rowptr, g.col, g.n, skip, fdeg);
-    g.h_fdeg.resize(g.n);
-    KZ_HIP(hipMemcpyAsync(g.h_fdeg.data(), fdeg, sizeof(uint32_t) * g.n, hipMemcpyDeviceToHost, s));
-    KZ_HIP(hipStreamSynchronize(s));
-    std::vector<uint64_t> frp(size_t(g.n) + 1, 0);
-    for (uint32_t v = 0; v < g.n; ++v) frp[v + 1] = frp[v] + g.h_fdeg[v];
-    KZ_HIP(hipMalloc(&g.fwd_rowptr, sizeof(uint64_t) * frp.size()));
-    KZ_HIP(hipMalloc(&g.fwd_col, col_alloc_bytes(frp[g.n])));
-    KZ_HIP(hipMemcpyAsync(g.fwd_rowptr, frp.data(), sizeof(uint64_t) * frp.size(), hipMemcpyHostToDevice, s));
-    if (g.n)
-        hipLaunchKernelGGL(fwd_copy_kernel, dim3((uint64_t(g.n) * 64 + 255) / 256), dim3(256), 0, s, g.rowptr, g.col,
-                           g.n, skip, g.fwd_rowptr, g.fwd_col);
-    KZ_HIP(hipStreamSynchronize(s));
-    hipFree(skip);
-    hipFree(fdeg);
+    // Counts do not depend on the vertex order; ranking by degree bounds
+    // every forward degree by sqrt(2m) (hubs keep few forward neighbours).
+    uint64_t fnnz = 0;
+    if (int e = orient_resident(g, &g.fwd_rowptr, &g.fwd_col, nullptr, &g.h_fdeg, &fnnz)) return e;
+    g.fwd_bytes = 8ull * (uint64_t(g.n) + 1) + col_alloc_bytes(fnnz);
+    g.fwd_max = 0;
+    for (uint32_t x : g.h_fdeg) g.fwd_max = std::max(g.fwd_max, x);
     g.fwd_ready = true;
     return KZ_OK;
+}
+
+void release_forward(DeviceGraph& g) {
+    if (!g.fwd_ready || g.oriented) return;
+    if (rt().ready) (void)hipStreamSynchronize(rt().stream);
+    (void)hipFree(g.fwd_rowptr);
+    (void)hipFree(g.fwd_col);
+    g.fwd_rowptr = nullptr;
+    g.fwd_col = nullptr;
+    g.h_fdeg.clear();
+    g.h_fdeg.shrink_to_fit();
+    g.fwd_bytes = 0;
+    g.fwd_max = 0;
+    g.fwd_ready = false;
 }
 
 static BucketCfg graph_buckets(const DeviceGraph& d, int k) {
@@ -154,6 +158,13 @@ int kz_count_cliques(kz_graph g, int k, int method, uint64_t* count, kz_run_stat
     double dev_ms = 0;
     kz_run_stats st;
     std::memset(&st, 0, sizeof(st));
+    if (method == KZ_METHOD_PERSISTENT) {
+        if (int e = ensure_forward(d)) return e;
+        // a root whose forward list cannot be staged in LDS even after the
+        // orientation (a clique core of more than kMaxBitmapDegree vertices)
+        // is counted by the plain form, which has no degree limit
+        if (d.fwd_max > kMaxBitmapDegree) method = KZ_METHOD_PLAIN;
+    }
     if (method == KZ_METHOD_PERSISTENT) {
         CliqueJob* j = nullptr;
         if (int e = graph_job(d, k, &j)) return e;

@@ -55,12 +55,16 @@ struct DeviceGraph {
     // host copies of the degree array (for bucketing and plan sizing)
     std::vector<uint32_t> h_deg;
     uint64_t bytes = 0;
-    // forward part (v > u) of a symmetric graph, built on first use by the
-    // clique kernels; for oriented graphs these alias rowptr/col.
+    // forward part of a symmetric graph for the clique kernels, built on
+    // first use: vertices renumbered by (degree, id) rank, u -> v kept for
+    // rank v > rank u (fwd_bytes of device memory); for oriented graphs
+    // these alias rowptr/col.
     uint64_t* fwd_rowptr = nullptr;
     uint32_t* fwd_col = nullptr;
     std::vector<uint32_t> h_fdeg;
     bool fwd_ready = false;
+    uint64_t fwd_bytes = 0;
+    uint32_t fwd_max = 0;  // largest forward out-degree
     // prepared persistent clique executions, indexed by k (3..5), and the
     // degree-bucket configuration each uses (set or autotuned)
     std::shared_ptr<void> clique_exec[6];

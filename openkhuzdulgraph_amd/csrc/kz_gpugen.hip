@@ -113,6 +113,26 @@ struct ChungLuSource {  // Chung-Lu draws (kz_gen_chung_lu) against the host's t
     }
 };
 
+struct CsrForwardSource {  // forward keys of a resident symmetric CSR
+    const uint64_t* rowptr;
+    const uint32_t* col;
+    const uint32_t* rank;
+    uint32_t n;
+    __device__ bool operator()(uint64_t j, uint64_t* out) const {
+        // row u of entry j: rowptr[lo] <= j < rowptr[hi] holds throughout
+        uint32_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (rowptr[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t ru = rank[lo], rv = rank[col[j]];
+        if (rv <= ru) return false;
+        *out = (uint64_t(ru) << 32) | rv;
+        return true;
+    }
+};
+
 struct ForwardSource {  // forward keys of the cleaned graph (orientation)
     const uint64_t* keys;  // sorted unique (u << 32 | v), original ids
     const uint32_t* newid;
@@ -299,6 +319,21 @@ __global__ void rank_from_sorted_kernel(const uint64_t* __restrict__ sorted, uin
         rank[uint32_t(sorted[i])] = i;
 }
 
+// Wave per vertex u: fdeg[rank u] = |{v in N(u) : rank v > rank u}|.
+__global__ void rank_fdeg_kernel(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                 const uint32_t* __restrict__ rank, uint32_t n, uint32_t* __restrict__ fdeg) {
+    const uint32_t u = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    if (u >= n) return;
+    const uint64_t b = rowptr[u];
+    const uint32_t d = uint32_t(rowptr[u + 1] - b);
+    const uint32_t ru = rank[u];
+    uint32_t c = 0;
+    for (uint32_t k = lane; k < d; k += 64) c += rank[col[b + k]] > ru ? 1u : 0u;
+    for (int off = 32; off; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) fdeg[ru] = c;
+}
+
 __global__ void bounds_degree_kernel(const uint64_t* __restrict__ start, const uint64_t* __restrict__ end, uint32_t n,
                                      uint32_t* __restrict__ deg) {
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
@@ -471,6 +506,76 @@ int build_graph(const Src& src, uint64_t draws, uint32_t n_hint, bool orient, kz
 }
 
 }  // namespace
+
+// Degree orientation of a resident symmetric graph
+// (toolkits/to_oriented_dataset_converter.cc:27-124): vertices ranked by
+// (degree, id), u -> v kept when rank v > rank u, ids replaced by ranks and
+// every list sorted -- the CSR that kz_orient_by_degree builds on the host.
+// The outputs are written only on success.
+int orient_resident(const DeviceGraph& g, uint64_t** rowptr_out, uint32_t** col_out, uint32_t** deg_out,
+                    std::vector<uint32_t>* h_deg, uint64_t* nnz_out) {
+    hipStream_t s = rt().stream;
+    const uint32_t n = g.n;
+    if (n >= (1u << 31)) KZ_FAIL(KZ_ERR_UNSUPPORTED, "orientation supports < 2^31 vertices");
+    const uint32_t grid = uint32_t(std::max(1, rt().num_cus * 4));
+    Scratch tmp;
+    struct ScratchFree {
+        Scratch& t;
+        ~ScratchFree() {
+            if (t.ptr) (void)hipFree(t.ptr);
+        }
+    } tmp_free{tmp};
+    DevArray<uint64_t> rk, rs, rowptr;
+    DevArray<uint32_t> rank, fdeg, col;
+    if (int e = rk.alloc(8ull * n, "rank keys")) return e;
+    if (int e = rs.alloc(8ull * n, "sorted rank keys")) return e;
+    if (int e = rank.alloc(4ull * n, "ranks")) return e;
+    if (int e = fdeg.alloc(4ull * (uint64_t(n) + 1), "out-degrees")) return e;
+    if (int e = rowptr.alloc(8ull * (uint64_t(n) + 1), "rowptr")) return e;
+    KZ_HIP(hipMemsetAsync(fdeg.p + n, 0, 4, s));
+    if (n) {
+        hipLaunchKernelGGL(rank_key_kernel, dim3(grid), dim3(256), 0, s, g.deg, n, rk.p);
+        size_t bytes = 0;
+        KZ_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, rk.p, rs.p, int(n), 0, 64, s));
+        if (int e = scratch_reserve(tmp, bytes)) return e;
+        KZ_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.ptr, bytes, rk.p, rs.p, int(n), 0, 64, s));
+        hipLaunchKernelGGL(rank_from_sorted_kernel, dim3(grid), dim3(256), 0, s, rs.p, n, rank.p);
+        hipLaunchKernelGGL(rank_fdeg_kernel, dim3((uint64_t(n) * 64 + 255) / 256), dim3(256), 0, s, g.rowptr, g.col,
+                           rank.p, n, fdeg.p);
+    }
+    if (int e = scan_u32_to_u64(fdeg.p, rowptr.p, uint64_t(n) + 1, tmp, s)) return e;
+    uint64_t fnnz = 0;
+    KZ_HIP(hipMemcpyAsync(&fnnz, rowptr.p + n, 8, hipMemcpyDeviceToHost, s));
+    KZ_HIP(hipStreamSynchronize(s));
+    if (int e = col.alloc(col_alloc_bytes(fnnz), "forward lists")) return e;
+    if (fnnz) {
+        Buffers f;
+        if (hipMalloc(&f.keys, 8 * fnnz) != hipSuccess ||
+            hipMalloc(&f.tmp, 8 * std::min(fnnz, g_chunk_items)) != hipSuccess ||
+            hipMalloc(&f.hist, 8 * kBuckets) != hipSuccess || hipMalloc(&f.nsel, 8) != hipSuccess) {
+            (void)hipGetLastError();
+            KZ_FAIL(KZ_ERR_NOMEM, "orientation keys for %llu edges", (unsigned long long)fnnz);
+        }
+        const CsrForwardSource src{g.rowptr, g.col, rank.p, n};
+        uint64_t mo = 0;
+        if (int e = bucketed_sort(src, g.nnz, 32 + bits_for(n ? n - 1 : 0), false, f, fnnz, &mo)) return e;
+        if (mo != fnnz) KZ_FAIL(KZ_ERR_INVALID, "orientation kept %llu of %llu forward entries",
+                                (unsigned long long)mo, (unsigned long long)fnnz);
+        hipLaunchKernelGGL(low_word_col_kernel, dim3(grid), dim3(256), 0, s, f.keys, fnnz, col.p);
+        KZ_HIP(hipGetLastError());
+        KZ_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<uint32_t> hd(n);
+    if (n) KZ_HIP(hipMemcpyAsync(hd.data(), fdeg.p, 4ull * n, hipMemcpyDeviceToHost, s));
+    KZ_HIP(hipStreamSynchronize(s));
+    *rowptr_out = rowptr.keep();
+    *col_out = col.keep();
+    if (deg_out) *deg_out = fdeg.keep();
+    h_deg->swap(hd);
+    *nnz_out = fnnz;
+    return KZ_OK;
+}
+
 }  // namespace kz
 
 using namespace kz;
@@ -514,6 +619,25 @@ int kz_gpu_gen_chung_lu(uint32_t n, uint64_t num_draws, double exponent, uint32_
 int kz_set_sort_chunk(uint64_t items) {
     if (items < 1024 || items > (uint64_t(1) << 30)) KZ_FAIL(KZ_ERR_INVALID, "sort chunk must be in [2^10, 2^30]");
     g_chunk_items = items;
+    return KZ_OK;
+}
+
+int kz_graph_orient(kz_graph sym, kz_graph* out) {
+    if (!sym || !out) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    if (sym->g.oriented) KZ_FAIL(KZ_ERR_INVALID, "the graph is already oriented");
+    if (int e = ensure_ready()) return e;
+    kz_graph g = new (std::nothrow) kz_graph_s();
+    if (!g) KZ_FAIL(KZ_ERR_NOMEM, "graph handle");
+    DeviceGraph& d = g->g;
+    if (int e = orient_resident(sym->g, &d.rowptr, &d.col, &d.deg, &d.h_deg, &d.nnz)) {
+        delete g;
+        return e;
+    }
+    d.n = sym->g.n;
+    d.oriented = 1;
+    for (uint32_t x : d.h_deg) d.max_degree = std::max(d.max_degree, x);
+    d.bytes = 8ull * (uint64_t(d.n) + 1) + col_alloc_bytes(d.nnz) + 4ull * (uint64_t(d.n) + 1);
+    *out = g;
     return KZ_OK;
 }
 

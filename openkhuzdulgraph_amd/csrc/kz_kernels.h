@@ -215,7 +215,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // Graph view the clique kernels run on: the forward (u -> v, v > u) part of
-// the graph, which is the graph itself when it is degree-oriented.
+// a degree-oriented graph.
 struct CliqueView {
     const uint64_t* rowptr = nullptr;
     const uint32_t* col = nullptr;
@@ -301,6 +301,17 @@ int graph_job(DeviceGraph& d, int k, CliqueJob** out);
 int motif_census(DeviceGraph& d, uint64_t* counts, kz_run_stats* stats);
 uint64_t census_bytes(const DeviceGraph& d);
 int clique_workload(const CliqueView& gv, uint64_t* out6);
+// Forward part of a graph for the clique kernels: the graph itself when it
+// is degree-oriented, else its orientation by (degree, id) rank, built once
+// and kept in the graph (fwd_bytes of device memory).
 int ensure_forward(DeviceGraph& g);
+// Frees the forward copy of a symmetric graph (no-op for oriented graphs);
+// no clique execution on the graph may be left.
+void release_forward(DeviceGraph& g);
+// Degree orientation of a resident symmetric graph (kz_gpugen.hip): ids
+// become ranks by (degree, id), u -> v kept for rank v > rank u, lists
+// sorted; outputs written only on success (deg_out may be null).
+int orient_resident(const DeviceGraph& g, uint64_t** rowptr_out, uint32_t** col_out, uint32_t** deg_out,
+                    std::vector<uint32_t>* h_deg, uint64_t* nnz_out);
 
 }  // namespace kz

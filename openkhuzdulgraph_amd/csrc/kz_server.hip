@@ -89,10 +89,26 @@ Prepared* find_prepared(kz_server_s* s, kz_graph g, int kind, int k) {
     return nullptr;
 }
 
+// Kind of the Prepared entry that accounts for the forward copy of a
+// symmetric graph, which every clique execution on that graph runs on.
+constexpr int kForwardCopy = -1;
+
+bool clique_exec_on(const DeviceGraph& d) {
+    for (int k = 3; k <= 5; ++k)
+        if (d.clique_exec[k]) return true;
+    return false;
+}
+
 void release_prepared(kz_server_s* s, Prepared* p) {
     DeviceGraph& d = p->g->g;
     if (p->kind == KZ_QUERY_CLIQUE) d.clique_exec[p->k].reset();
-    else d.census.reset();
+    else if (p->kind == kForwardCopy) {
+        // still in use by a clique execution (one being prepared): freed
+        // with it
+        if (!clique_exec_on(d)) release_forward(d);
+    } else {
+        d.census.reset();
+    }
     s->in_use -= p->bytes;
     for (size_t i = 0; i < s->prepared.size(); ++i)
         if (s->prepared[i] == p) {
@@ -102,19 +118,25 @@ void release_prepared(kz_server_s* s, Prepared* p) {
     delete p;
 }
 
+// Device bytes the degree orientation of a symmetric graph takes while it
+// is built: ranks and sort keys, the forward lists (half the entries) and
+// the keys of the bucketed sort (16 bytes per forward entry at most).
+uint64_t forward_estimate(const DeviceGraph& d) {
+    return 8ull * (d.n + 1) + 28ull * d.n + col_alloc_bytes(d.nnz) + 8ull * d.nnz + (1u << 20);
+}
+
 // Device bytes a not yet prepared execution will take (upper estimates).
 uint64_t estimate_bytes(const DeviceGraph& d, int kind, int k) {
     if (kind == KZ_QUERY_CLIQUE) {
         // task lists and cursors; for k >= 4 the hub roots' bitmap rows in
         // HBM (d * ceil(d / 64) words each, out-degrees above the default
-        // 1024 bound; a symmetric graph's degrees bound its out-degrees);
-        // the forward copy of a symmetric graph
+        // 1024 bound); a symmetric graph's forward copy is accounted apart
+        const std::vector<uint32_t>& fd = d.oriented ? d.h_deg : d.h_fdeg;
         uint64_t hubs = 0;
         if (k >= 4)
-            for (uint32_t x : d.h_deg)
+            for (uint32_t x : fd)
                 if (x > 1024) hubs += uint64_t(x) * ((x + 63) / 64) * 8;
-        return 8ull * d.n + (1u << 20) + hubs +
-               (d.oriented || d.fwd_ready ? 0 : 8ull * (d.n + 1) + col_alloc_bytes(d.nnz));
+        return 8ull * d.n + (1u << 20) + hubs;
     }
     // relabelled copy, forward part, edge counters, 4-cycle slabs (at most
     // 2 workgroups per CU) and the 4-clique job
@@ -134,6 +156,13 @@ uint64_t free_device_bytes() {
     return f;
 }
 
+// An idle execution can be released; a forward copy once no clique
+// execution on its graph is left.
+bool releasable(const Prepared* p) {
+    if (p->busy) return false;
+    return p->kind != kForwardCopy || !clique_exec_on(p->g->g);
+}
+
 // Frees idle prepared executions, least recently used first, until `need`
 // more bytes fit in the budget and in the device's free memory.  Returns
 // whether they fit.
@@ -142,7 +171,7 @@ bool make_room(kz_server_s* s, uint64_t need) {
     while (s->in_use + need > B || (need && free_device_bytes() < need + kFreeMargin)) {
         Prepared* victim = nullptr;
         for (Prepared* p : s->prepared)
-            if (!p->busy && (!victim || p->last_use < victim->last_use)) victim = p;
+            if (releasable(p) && (!victim || p->last_use < victim->last_use)) victim = p;
         if (!victim) return false;
         release_prepared(s, victim);
     }
@@ -168,7 +197,7 @@ bool release_idle(kz_server_s* s, Prepared* keep) {
     for (bool again = true; again;) {
         again = false;
         for (Prepared* p : s->prepared)
-            if (p != keep && !p->busy) {
+            if (p != keep && releasable(p)) {
                 release_prepared(s, p);
                 any = again = true;
                 break;
@@ -195,6 +224,30 @@ bool account(kz_server_s* s, Prepared* p, uint64_t bytes) {
     return fits;
 }
 
+// Accounts the forward copy of a symmetric graph (built by graph_job) once
+// per graph.  Returns false when it does not fit the budget; the entry is
+// then dropped.
+bool adopt_forward(kz_server_s* s, kz_graph g) {
+    DeviceGraph& d = g->g;
+    if (d.oriented || !d.fwd_ready || !d.fwd_bytes) return true;
+    if (Prepared* f = find_prepared(s, g, kForwardCopy, 0)) {
+        f->last_use = ++s->tick;
+        return true;
+    }
+    Prepared* f = new Prepared{g, kForwardCopy, 0, 0, ++s->tick, 0};
+    s->prepared.push_back(f);
+    return account(s, f, d.fwd_bytes);
+}
+
+// Drops a clique execution that could not be accounted, and the graph's
+// forward copy when nothing uses or accounts for it any more.
+void drop_clique(kz_server_s* s, Prepared* p) {
+    const kz_graph g = p->g;
+    release_prepared(s, p);
+    DeviceGraph& d = g->g;
+    if (d.fwd_ready && !clique_exec_on(d) && !find_prepared(s, g, kForwardCopy, 0)) release_forward(d);
+}
+
 // Prepares (if needed) and launches or runs one query.  Returns false when
 // it must wait for memory held by running queries.
 bool start_query(kz_server_s* s, SQuery* q, int slot) {
@@ -209,19 +262,52 @@ bool start_query(kz_server_s* s, SQuery* q, int slot) {
             s->prepared.push_back(p);
             const uint64_t bytes =
                 q->kind == KZ_QUERY_CLIQUE ? static_cast<CliqueJob*>(d.clique_exec[q->k].get())->bytes : census_bytes(d);
-            if (!account(s, p, bytes)) {
+            p->busy++;
+            const bool fwd_fits = q->kind != KZ_QUERY_CLIQUE || adopt_forward(s, q->g);
+            p->busy--;
+            if (!fwd_fits) drop_clique(s, p);
+            if (!fwd_fits || !account(s, p, bytes)) {
                 if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
                 return !any_busy(s);
             }
         }
     }
-    if (!p) {
-        const uint64_t est = estimate_bytes(d, q->kind, q->k);
+    if (!p && q->kind == KZ_QUERY_CLIQUE && !d.oriented && !d.fwd_ready) {
+        // a symmetric graph's first clique query: its degree orientation
+        // first (accounted with the graph), then the job, sized from the
+        // exact forward degrees
+        const uint64_t est = forward_estimate(d);
         if (est > budget_of(s)) {
             finish(s, q, KZ_ERR_BUDGET);
             return true;
         }
         if (!make_room(s, est)) {
+            if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
+            return !any_busy(s);
+        }
+        int e = ensure_forward(d);
+        if ((e == KZ_ERR_DEVICE || e == KZ_ERR_NOMEM) && release_idle(s, nullptr)) e = ensure_forward(d);
+        if (e) {
+            finish(s, q, e);
+            return true;
+        }
+        if (!adopt_forward(s, q->g)) {
+            release_forward(d);
+            if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
+            return !any_busy(s);
+        }
+    }
+    if (!p) {
+        const uint64_t est = estimate_bytes(d, q->kind, q->k);
+        Prepared* fwd = find_prepared(s, q->g, kForwardCopy, 0);
+        if (fwd) fwd->busy++;  // the job needs it: not a victim for its room
+        const bool room = est <= budget_of(s) && make_room(s, est);
+        if (fwd) fwd->busy--;
+        if (est > budget_of(s)) {
+            finish(s, q, KZ_ERR_BUDGET);
+            return true;
+        }
+        if (!room) {
             // nothing running would free more: it cannot fit
             if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
             return !any_busy(s);
@@ -229,7 +315,6 @@ bool start_query(kz_server_s* s, SQuery* q, int slot) {
         p = new Prepared{q->g, q->kind, q->k, 0, 0, 0};
         s->prepared.push_back(p);
         if (q->kind == KZ_QUERY_CLIQUE) {
-            const uint64_t fwd = (d.oriented || d.fwd_ready) ? 0 : 8ull * (d.n + 1) + col_alloc_bytes(d.nnz);
             CliqueJob* job = nullptr;
             int e = graph_job(d, q->k, &job);
             // out of device memory despite the estimate: release every idle
@@ -240,7 +325,13 @@ bool start_query(kz_server_s* s, SQuery* q, int slot) {
                 finish(s, q, e);
                 return true;
             }
-            if (!account(s, p, job->bytes + fwd)) {
+            // the forward copy the job runs on belongs to the graph and is
+            // accounted once for all its clique executions
+            p->busy++;
+            const bool fwd_fits = adopt_forward(s, q->g);
+            p->busy--;
+            if (!fwd_fits) drop_clique(s, p);
+            if (!fwd_fits || !account(s, p, job->bytes)) {
                 if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
                 return !any_busy(s);
             }
