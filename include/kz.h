@@ -248,6 +248,43 @@ int kz_batch_release(kz_batch b);
  * reference reports no bandwidth for this path. */
 int kz_clique_workload(kz_graph g, uint64_t* out6);
 
+/* ---- Graphs beyond HBM ---------------------------------------------------
+ * A degree-oriented CSR whose neighbour lists stay in pinned host memory
+ * (hipHostMalloc; the row offsets and degrees, about 36 bytes per vertex
+ * with the working arrays, are in HBM), counted in batches of consecutive
+ * roots.  Per batch the lists the batch needs -- its roots' and their
+ * forward neighbours' -- are requested once per vertex (a device-side
+ * deduplicated request list), served from an HBM cache of the
+ * highest-degree vertices' lists or read from host memory over PCIe into a
+ * working CSR of list_bytes, and the batch's roots are counted there by the
+ * persistent clique kernels.  Replaces the remote graph-data fetch of the
+ * distributed engine: GraphDataRequestSender / its handler with the cache of
+ * high-degree lists (src/engine.cc:94-510, :513-647; cache_degree_threshold
+ * and relative_cache_size of DistributedApplication,
+ * include/distributed_application.h:38-46).  A root whose own lists exceed
+ * list_bytes fails kz_paged_create with KZ_ERR_BUDGET. */
+typedef struct kz_paged_s* kz_paged;
+typedef struct kz_paged_cfg {
+    uint64_t list_bytes;              /* HBM for the lists of one root batch  */
+    uint64_t cache_bytes;             /* HBM for the cache of lists           */
+    uint32_t cache_degree_threshold;  /* total degree for a list to be cached */
+    uint32_t reserved;
+} kz_paged_cfg;
+typedef struct kz_paged_stats {
+    double device_ms;          /* HIP-event time of all batches          */
+    double wall_ms;
+    uint64_t batches;
+    uint64_t requests;         /* distinct vertices fetched, all batches */
+    uint64_t request_refs;     /* list references before deduplication   */
+    uint64_t host_bytes;       /* list bytes read from host memory       */
+    uint64_t cache_bytes;      /* list bytes served by the HBM cache     */
+    uint64_t cached_vertices;  /* lists the cache holds                  */
+} kz_paged_stats;
+int kz_paged_create(const kz_csr* oriented, const kz_paged_cfg* cfg, kz_paged* out);
+/* Exact k-clique count (k = 3..5), as kz_count_cliques. */
+int kz_paged_count_cliques(kz_paged p, int k, uint64_t* count, kz_paged_stats* stats);
+int kz_paged_release(kz_paged p);
+
 /* ---- Motif census -------------------------------------------------------
  * Induced connected 3- and 4-vertex subgraph counts of a resident SYMMETRIC
  * graph, counts[0..8) = {wedge (open 2-path), triangle, 3-path, 3-star,

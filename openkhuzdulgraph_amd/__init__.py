@@ -24,7 +24,7 @@ from ._ffi import KzError, KzLibraryMissing, check, lib
 __all__ = [
     "Batch", "CSRGraph", "DeviceGraph", "KzError", "KzLibraryMissing", "init", "shutdown", "device_info",
     "gen_erdos_renyi", "gen_rmat", "gen_chung_lu", "from_edges", "load_dgraph", "set_level_budget",
-    "clique_plan", "Server", "gpu_gen_erdos_renyi", "gpu_gen_rmat", "gpu_gen_chung_lu",
+    "clique_plan", "Server", "gpu_gen_erdos_renyi", "gpu_gen_rmat", "gpu_gen_chung_lu", "PagedGraph",
 ]
 
 _METHODS = {"plain": _ffi.KZ_METHOD_PLAIN, "persistent": _ffi.KZ_METHOD_PERSISTENT}
@@ -223,6 +223,38 @@ def set_op_batch(op: str, lists, pairs, bounds=None):
     if code in (_ffi.KZ_SETOP_INTERSECT, _ffi.KZ_SETOP_SUBTRACT):
         return [outv[int(oo[i]):int(oo[i]) + int(outl[i])].copy() for i in range(len(pairs))]
     return outl[:len(pairs)].copy()
+
+
+class PagedGraph:
+    """A degree-oriented graph whose neighbour lists stay in pinned host
+    memory, counted in root batches with lists fetched into HBM on demand
+    (kz_paged_*): list_bytes of HBM for one batch's lists, cache_bytes for
+    the lists of the highest-degree vertices (total degree >=
+    cache_degree_threshold)."""
+
+    def __init__(self, graph: CSRGraph, list_bytes: int, cache_bytes: int = 0, cache_degree_threshold: int = 64):
+        cfg = _ffi.kz_paged_cfg(int(list_bytes), int(cache_bytes), int(cache_degree_threshold), 0)
+        self._h = C.c_void_p()
+        check(lib().kz_paged_create(C.byref(graph._raw), C.byref(cfg), C.byref(self._h)))
+        self.last_stats: dict = {}
+
+    def count_cliques(self, k: int) -> int:
+        out = C.c_uint64()
+        st = _ffi.kz_paged_stats()
+        check(lib().kz_paged_count_cliques(self._h, k, C.byref(out), C.byref(st)))
+        self.last_stats = st.as_dict()
+        return int(out.value)
+
+    def release(self) -> None:
+        if self._h:
+            check(lib().kz_paged_release(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 class Batch:

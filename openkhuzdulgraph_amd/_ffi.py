@@ -112,6 +112,21 @@ class kz_bucket_cfg(C.Structure):
         return {"bound": list(self.bound), "block": list(self.block), "serial": int(self.serial)}
 
 
+class kz_paged_cfg(C.Structure):
+    _fields_ = [("list_bytes", C.c_uint64), ("cache_bytes", C.c_uint64), ("cache_degree_threshold", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class kz_paged_stats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("wall_ms", C.c_double), ("batches", C.c_uint64),
+                ("requests", C.c_uint64), ("request_refs", C.c_uint64), ("host_bytes", C.c_uint64),
+                ("cache_bytes", C.c_uint64), ("cached_vertices", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {name: (float(getattr(self, name)) if name.endswith("_ms") else int(getattr(self, name)))
+                for name, _ in self._fields_}
+
+
 KZ_FSM_LEVELS = 6
 FSM_LEVELS = ("vertex", "edge", "wedge", "triangle", "four_chain", "three_star")
 
@@ -221,6 +236,9 @@ _SIGS = {
                                       C.POINTER(C.c_void_p)]),
     "kz_graph_download": (C.c_int, [C.c_void_p, C.POINTER(kz_csr)]),
     "kz_graph_orient": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "kz_paged_create": (C.c_int, [C.POINTER(kz_csr), C.POINTER(kz_paged_cfg), C.POINTER(C.c_void_p)]),
+    "kz_paged_count_cliques": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(kz_paged_stats)]),
+    "kz_paged_release": (C.c_int, [C.c_void_p]),
     "kz_graph_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_int)]),
     "kz_set_sort_chunk": (C.c_int, [C.c_uint64]),

@@ -30,7 +30,7 @@ ARCH = "gfx950"
 HOST_SOURCES = ["kz_graph_host.cpp"]
 HIP_SOURCES = ["kz_runtime.hip", "kz_clique.hip", "kz_plan.hip", "kz_count.hip", "kz_setops.hip", "kz_workload.hip",
                "kz_census.hip", "kz_server.hip",
-               "kz_gpugen.hip", "kz_fsm.hip"]
+               "kz_gpugen.hip", "kz_fsm.hip", "kz_paged.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

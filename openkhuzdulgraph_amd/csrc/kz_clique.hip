@@ -869,7 +869,8 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         const CliqueQuery& Q = queries[qi];
         if (Q.k < 3 || Q.k > 5) KZ_FAIL(KZ_ERR_UNSUPPORTED, "clique size %d outside 3..5", Q.k);
         const std::vector<uint32_t>& fd = *Q.fdeg;
-        for (uint32_t v = 0; v < fd.size(); ++v) {
+        const uint32_t vend = uint32_t(std::min<uint64_t>(fd.size(), Q.root_hi));
+        for (uint32_t v = Q.root_lo; v < vend; ++v) {
             const uint32_t d = fd[v];
             if (d < uint32_t(Q.k - 1)) continue;
             if (d > D)
@@ -902,7 +903,8 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
     for (uint32_t qi = 0; qi < queries.size(); ++qi) {
         const CliqueQuery& Q = queries[qi];
         const std::vector<uint32_t>& fd = *Q.fdeg;
-        for (uint32_t v = 0; v < fd.size(); ++v) {
+        const uint32_t vend = uint32_t(std::min<uint64_t>(fd.size(), Q.root_hi));
+        for (uint32_t v = Q.root_lo; v < vend; ++v) {
             const uint32_t d = fd[v];
             if (d < uint32_t(Q.k - 1)) continue;
             const uint64_t at = pos[list_of(Q.k, d)][d]++;

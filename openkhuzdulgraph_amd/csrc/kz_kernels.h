@@ -234,12 +234,14 @@ BucketCfg default_buckets();
 int check_buckets(const BucketCfg& c);
 std::vector<BucketCfg> tuning_candidates();
 
-// One counting query of the persistent form: k-cliques of one forward graph.
+// One counting query of the persistent form: k-cliques of one forward graph
+// rooted at the vertices in [root_lo, root_hi).
 struct CliqueQuery {
     const uint64_t* rowptr;
     const uint32_t* col;
     const std::vector<uint32_t>* fdeg;  // host forward out-degrees
     int k;
+    uint32_t root_lo = 0, root_hi = 0xFFFFFFFFu;
 };
 
 // Prepared persistent execution of a set of queries: task groups per

@@ -37,6 +37,7 @@ class DistributedApplication {
     ApplyFunction apply_fun_;
     long long memory_size_per_level_;
     void* device_graph_;  // kz_graph
+    void* paged_graph_;   // kz_paged: lists in host memory (beyond the resident budget)
     bool probe_only_;
     typedef void (*AddFn)(void*, uint64_t);
     std::vector<void*> aggregators_;  // in allocation order

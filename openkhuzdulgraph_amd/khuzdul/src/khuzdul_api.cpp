@@ -17,6 +17,7 @@
 #include <sys/stat.h>
 #include <sys/time.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -149,8 +150,6 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
                                                bool finer_grained_multithreading, long long memory_size_per_level,
                                                VertexId cache_degree_threshold, double relative_cache_size) {
     (void)finer_grained_multithreading;
-    (void)cache_degree_threshold;
-    (void)relative_cache_size;
     assert(max_pattern_size >= 3);  // distributed_application.cc:71
     if (max_pattern_size > MAX_PATTERN_SIZE) {
         std::fprintf(stderr, "pattern size %d exceeds MAX_PATTERN_SIZE %d\n", max_pattern_size, MAX_PATTERN_SIZE);
@@ -174,6 +173,7 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
     // every count is then reported as 0.
     probe_only_ = std::getenv("KZ_PROBE_ONLY") != nullptr;
     device_graph_ = nullptr;
+    paged_graph_ = nullptr;
     if (probe_only_) {
         std::printf("KZ_PROBE_ONLY: plans are extracted and dumped, nothing is counted\n");
         return;
@@ -194,6 +194,39 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
     graph_.num_edges_ = csr.num_entries;
     graph_.max_degree_ = csr.max_degree;
     graph_.oriented_ = csr.oriented != 0;
+    // KZ_RESIDENT_BYTES: HBM budget for the neighbour lists.  A graph whose
+    // lists exceed it keeps them in host memory and is counted in root
+    // batches (kz_paged_*), with cache_degree_threshold and
+    // relative_cache_size sizing the HBM cache of high-degree lists as the
+    // reference's remote-data cache (src/engine.cc:94-510).
+    const char* rb = std::getenv("KZ_RESIDENT_BYTES");
+    const uint64_t resident = rb ? std::strtoull(rb, nullptr, 10) : 0;
+    if (resident && 4 * csr.num_entries > resident) {
+        kz_csr ori;
+        std::memset(&ori, 0, sizeof(ori));
+        const kz_csr* src = &csr;
+        if (!csr.oriented) {
+            die_kz(kz_orient_by_degree(&csr, &ori), "kz_orient_by_degree");
+            src = &ori;
+        }
+        kz_paged_cfg cfg;
+        std::memset(&cfg, 0, sizeof(cfg));
+        cfg.list_bytes = resident;
+        cfg.cache_bytes = uint64_t(std::max(0.0, relative_cache_size) * 4.0 * double(src->num_entries));
+        cfg.cache_degree_threshold = cache_degree_threshold;
+        kz_paged pg = nullptr;
+        die_kz(kz_paged_create(src, &cfg, &pg), "kz_paged_create");
+        kz_csr_release(&ori);
+        kz_csr_release(&csr);
+        paged_graph_ = pg;
+        t += get_time();
+        LocalPerformanceMetric::load_graph_time = t;
+        std::printf("The neighbour lists exceed KZ_RESIDENT_BYTES (%llu): they stay in host memory, %llu bytes of "
+                    "HBM hold one root batch's lists and %llu the cache (%.3f s to load).\n",
+                    (unsigned long long)resident, (unsigned long long)cfg.list_bytes,
+                    (unsigned long long)cfg.cache_bytes, t);
+        return;
+    }
     kz_graph g = nullptr;
     die_kz(kz_graph_upload(&csr, &g), "kz_graph_upload");
     kz_csr_release(&csr);
@@ -205,7 +238,9 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
 
 DistributedApplication::~DistributedApplication() {
     if (device_graph_) kz_graph_release(static_cast<kz_graph>(device_graph_));
+    if (paged_graph_) kz_paged_release(static_cast<kz_paged>(paged_graph_));
     device_graph_ = nullptr;
+    paged_graph_ = nullptr;
 }
 
 void DistributedApplication::register_aggregator(void* a, AddFn add) {
@@ -275,7 +310,20 @@ void DistributedApplication::run(bool disable_output, bool restrict_starting_ver
     kz_plan_desc clique;
     bool is_clique = plan.num_aggregators == 1 && max_pattern_size_ <= 5 &&
                      kz_plan_clique(max_pattern_size_, &clique) == KZ_OK && same_plan(plan, clique);
-    if (is_clique && !force_plain) {
+    if (paged_graph_) {
+        if (!is_clique) {
+            std::fprintf(stderr, "a graph beyond KZ_RESIDENT_BYTES is counted by the clique apps only\n");
+            std::exit(-1);
+        }
+        uint64_t c = 0;
+        kz_paged_stats ps;
+        die_kz(kz_paged_count_cliques(static_cast<kz_paged>(paged_graph_), max_pattern_size_, &c, &ps),
+               "kz_paged_count_cliques");
+        counts[0] = c;
+        st.device_ms = ps.device_ms;
+        st.launches = ps.batches;
+        LocalPerformanceMetric::last_method = "paged clique kernels (lists fetched from host memory per root batch)";
+    } else if (is_clique && !force_plain) {
         uint64_t c = 0;
         die_kz(kz_count_cliques(g, max_pattern_size_, KZ_METHOD_PERSISTENT, &c, &st), "kz_count_cliques");
         counts[0] = c;
