@@ -408,6 +408,21 @@ int kz_set_op_batch(int op, const uint32_t* values, const uint64_t* list_off, ui
                     const uint32_t* a_idx, const uint32_t* b_idx, const uint32_t* bounds, uint32_t num_pairs,
                     uint32_t* out_values, const uint64_t* out_off, uint32_t* out_len);
 
+/* kz_set_op_batch without any device allocation: no reference counterpart.
+ * For callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, checks and results;
+ * the device staging of the batch (lists, offsets, pair indices, results:
+ * the sum of each region rounded up to 256 bytes) lives in a file-scope
+ * device array of KZ_SETOP_NOALLOC_MAX_BYTES that each device the library
+ * runs on holds until the process ends.  A larger batch fails with
+ * KZ_ERR_INVALID before any device work.  Calls share the array: the
+ * library serializes them within a process; each call keeps its staging
+ * until the next call overwrites it. */
+#define KZ_SETOP_NOALLOC_MAX_BYTES (32ull << 20)
+int kz_set_op_batch_noalloc(int op, const uint32_t* values, const uint64_t* list_off, uint32_t num_lists,
+                            const uint32_t* a_idx, const uint32_t* b_idx, const uint32_t* bounds, uint32_t num_pairs,
+                            uint32_t* out_values, const uint64_t* out_off, uint32_t* out_len);
+
 /* ---- Plan IR: a generated apply() schedule in data form ----------------
  * Replaces the ApplyFunction callback (include/engine.h:589) with a table the
  * GPU executes.  An embedding of size s (matched vertices m_0..m_{s-1}) is

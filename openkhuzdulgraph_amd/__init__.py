@@ -191,8 +191,10 @@ def set_level_budget(nbytes: int) -> None:
     check(lib().kz_set_level_budget(int(nbytes)))
 
 
-def set_op_batch(op: str, lists, pairs, bounds=None):
-    """Batched VertexSet operations on the GPU (kz_set_op_batch).
+def set_op_batch(op: str, lists, pairs, bounds=None, noalloc: bool = False):
+    """Batched VertexSet operations on the GPU (kz_set_op_batch, or with
+    noalloc=True kz_set_op_batch_noalloc: no device allocation, batches up
+    to KZ_SETOP_NOALLOC_MAX_BYTES of staging).
 
     op: "intersect" | "subtract" | "intersect_count" | "trim_lower_bound".
     lists: sequence of strictly increasing id sequences; pairs: sequence of
@@ -217,9 +219,10 @@ def set_op_batch(op: str, lists, pairs, bounds=None):
     outv = np.zeros(max(1, int(oo[-1])), np.uint32)
     outl = np.zeros(max(1, len(pairs)), np.uint32)
     P32, P64 = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
-    check(lib().kz_set_op_batch(code, vals.ctypes.data_as(P32), off.ctypes.data_as(P64), len(arrs),
-                                pa.ctypes.data_as(P32), pb.ctypes.data_as(P32), bd.ctypes.data_as(P32), len(pairs),
-                                outv.ctypes.data_as(P32), oo.ctypes.data_as(P64), outl.ctypes.data_as(P32)))
+    fn = lib().kz_set_op_batch_noalloc if noalloc else lib().kz_set_op_batch
+    check(fn(code, vals.ctypes.data_as(P32), off.ctypes.data_as(P64), len(arrs),
+             pa.ctypes.data_as(P32), pb.ctypes.data_as(P32), bd.ctypes.data_as(P32), len(pairs),
+             outv.ctypes.data_as(P32), oo.ctypes.data_as(P64), outl.ctypes.data_as(P32)))
     if code in (_ffi.KZ_SETOP_INTERSECT, _ffi.KZ_SETOP_SUBTRACT):
         return [outv[int(oo[i]):int(oo[i]) + int(outl[i])].copy() for i in range(len(pairs))]
     return outl[:len(pairs)].copy()

@@ -268,12 +268,16 @@ _SIGS = {
     "kz_set_op_batch": (C.c_int, [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.c_uint32,
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32,
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "kz_set_op_batch_noalloc": (C.c_int, [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.c_uint32,
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32,
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
 }
 
 KZ_SETOP_INTERSECT = 1
 KZ_SETOP_SUBTRACT = 2
 KZ_SETOP_INTERSECT_COUNT = 3
 KZ_SETOP_TRIM_LOWER_BOUND = 4
+KZ_SETOP_NOALLOC_MAX_BYTES = 32 << 20
 
 
 def lib() -> C.CDLL:
