@@ -281,7 +281,86 @@ __device__ __forceinline__ kz_v4i bits_to_bytes(uint32_t w, uint32_t off) {
     return r;
 }
 
-//
+// One 32 x 32 tile (I, J >= I) of A A^T masked by A: this lane's share of
+// sum A[i][j] * (A A^T)[i][j] over the tile.  rows is a word-major bitmap
+// (word kk of row i at rows[kk * ws + i]) in LDS or in HBM.  Call with
+// every lane of the wave.
+__device__ __forceinline__ uint32_t mfma4_tile(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t I,
+                                               uint32_t J) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nb = (d + 31) >> 5;
+    const uint32_t i = 32 * I + (lane & 31), j = 32 * J + (lane & 31);
+    // rows past d read row d - 1: their products land in tile rows and
+    // columns that the mask below leaves out
+    const uint32_t ic = i < d ? i : d - 1, jc = j < d ? j : d - 1;
+    const uint32_t hs = 16 * (lane >> 5);
+    // one 64-bit word = two 32-column blocks, read once, the next word's
+    // reads issued before this one's products; word J / 2 may start at its
+    // high block (J odd), the last word may end after its low block (nb odd)
+    kz_v16i c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) c[r] = 0;
+    const uint32_t wend = (nb + 1) >> 1;
+    uint32_t kw = J >> 1;
+    unsigned long long wi = rows[size_t(kw) * ws + ic], wj = rows[size_t(kw) * ws + jc];
+    if (J & 1u) {
+        const uint32_t hi_i = uint32_t(wi >> 32), hi_j = uint32_t(wj >> 32);
+        ++kw;
+        if (kw < wend) {
+            wi = rows[size_t(kw) * ws + ic];
+            wj = rows[size_t(kw) * ws + jc];
+        }
+        c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(hi_i, hs), bits_to_bytes(hi_j, hs), c, 0, 0, 0);
+    }
+    while (2 * kw + 1 < nb) {
+        const uint32_t lo_i = uint32_t(wi), hi_i = uint32_t(wi >> 32);
+        const uint32_t lo_j = uint32_t(wj), hi_j = uint32_t(wj >> 32);
+        ++kw;
+        if (kw < wend) {
+            wi = rows[size_t(kw) * ws + ic];
+            wj = rows[size_t(kw) * ws + jc];
+        }
+        c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(lo_i, hs), bits_to_bytes(lo_j, hs), c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(hi_i, hs), bits_to_bytes(hi_j, hs), c, 0, 0, 0);
+    }
+    if (2 * kw < nb)
+        c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi), hs), bits_to_bytes(uint32_t(wj), hs), c,
+                                                  0, 0, 0);
+    // mask: A[i_r][j] for this lane's 16 rows i_r and its column j, read as
+    // the 32-bit half of each row's word that holds bit j (entries of A A^T
+    // are at most d < 2^24: 24-bit multiply-adds)
+    uint32_t s = 0;
+    if (j < d) {
+        const uint32_t* cw = reinterpret_cast<const uint32_t*>(rows + size_t(j >> 6) * ws) + ((j >> 5) & 1u);
+        const uint32_t b5 = j & 31u;
+        const uint32_t ib = 32 * I + (lane >> 5) * 4;
+        if (32 * I + 32 <= d) {  // every row of the tile exists (wave-uniform)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
+                s = __umul24(__builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u), uint32_t(c[r])) + s;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
+                const uint32_t a = ir < d ? __builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u) : 0u;
+                s = __umul24(a, uint32_t(c[r])) + s;
+            }
+        }
+    }
+    return s;
+}
+
+// (I, J) of tile t = J (J + 1) / 2 + I, I <= J.
+__device__ __forceinline__ void tile_coords(uint32_t t, uint32_t* I, uint32_t* J) {
+    uint32_t j = uint32_t((sqrtf(8.0f * float(t) + 1.0f) - 1.0f) * 0.5f);
+    if (j * (j + 1) / 2 > t) --j;
+    if ((j + 1) * (j + 2) / 2 <= t) ++j;
+    *J = j;
+    *I = t - j * (j + 1) / 2;
+}
+
 // Tiles go to the workgroup's waves from a counter in LDS (ctr, zeroed
 // before a barrier), the next tile's atomic issued ahead: tile t = J (J + 1)
 // / 2 + I, so the costly tiles (small J: the most 64-bit words) come first
@@ -297,79 +376,9 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
     while (tile < ntiles) {
         uint32_t tn = 0;
         if (lane == 0) tn = atomicAdd(ctr, 1u);
-        // J: the largest with J (J + 1) / 2 <= tile
-        uint32_t J = uint32_t((sqrtf(8.0f * float(tile) + 1.0f) - 1.0f) * 0.5f);
-        if (J * (J + 1) / 2 > tile) --J;
-        if ((J + 1) * (J + 2) / 2 <= tile) ++J;
-        const uint32_t I = tile - J * (J + 1) / 2;
-        {
-            const uint32_t i = 32 * I + (lane & 31), j = 32 * J + (lane & 31);
-            // rows past d read row d - 1: their products land in tile rows
-            // and columns that the mask below leaves out
-            const uint32_t ic = i < d ? i : d - 1, jc = j < d ? j : d - 1;
-            const uint32_t hs = 16 * (lane >> 5);
-            // one 64-bit word = two 32-column blocks, read once, the next
-            // word's reads issued before this one's products; word J / 2 may
-            // start at its high block (J odd), the last word may end after
-            // its low block (nb odd)
-            kz_v16i c;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) c[r] = 0;
-            const uint32_t wend = (nb + 1) >> 1;
-            uint32_t kw = J >> 1;
-            unsigned long long wi = rows[size_t(kw) * ws + ic], wj = rows[size_t(kw) * ws + jc];
-            if (J & 1u) {
-                const uint32_t hi_i = uint32_t(wi >> 32), hi_j = uint32_t(wj >> 32);
-                ++kw;
-                if (kw < wend) {
-                    wi = rows[size_t(kw) * ws + ic];
-                    wj = rows[size_t(kw) * ws + jc];
-                }
-                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(hi_i, hs), bits_to_bytes(hi_j, hs), c, 0, 0,
-                                                          0);
-            }
-            while (2 * kw + 1 < nb) {
-                const uint32_t lo_i = uint32_t(wi), hi_i = uint32_t(wi >> 32);
-                const uint32_t lo_j = uint32_t(wj), hi_j = uint32_t(wj >> 32);
-                ++kw;
-                if (kw < wend) {
-                    wi = rows[size_t(kw) * ws + ic];
-                    wj = rows[size_t(kw) * ws + jc];
-                }
-                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(lo_i, hs), bits_to_bytes(lo_j, hs), c, 0, 0,
-                                                          0);
-                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(hi_i, hs), bits_to_bytes(hi_j, hs), c, 0, 0,
-                                                          0);
-            }
-            if (2 * kw < nb)
-                c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi), hs),
-                                                          bits_to_bytes(uint32_t(wj), hs), c, 0, 0, 0);
-            // mask: A[i_r][j] for this lane's 16 rows i_r and its column j,
-            // read as the 32-bit half of each row's word that holds bit j
-            // (entries of A A^T are at most d < 2^24: 24-bit multiply-adds)
-            if (j < d) {
-                const uint32_t* cw =
-                    reinterpret_cast<const uint32_t*>(rows + size_t(j >> 6) * ws) + ((j >> 5) & 1u);
-                const uint32_t b5 = j & 31u;
-                const uint32_t ib = 32 * I + (lane >> 5) * 4;
-                uint32_t s = 0;
-                if (32 * I + 32 <= d) {  // every row of the tile exists (wave-uniform)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
-                        s = __umul24(__builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u), uint32_t(c[r])) + s;
-                    }
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
-                        const uint32_t a = ir < d ? __builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u) : 0u;
-                        s = __umul24(a, uint32_t(c[r])) + s;
-                    }
-                }
-                acc += s;
-            }
-        }
+        uint32_t I, J;
+        tile_coords(tile, &I, &J);
+        acc += mfma4_tile(rows, ws, d, I, J);
         tile = wave_bcast(tn);
     }
 }
@@ -617,6 +626,34 @@ __global__ __launch_bounds__(256) void hub_count_kernel(const HubMeta* __restric
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
 
+// 4-cliques of the hubs on MFMA: one wave per 32 x 32 tile of a hub's
+// A A^T, the tiles of every hub (tile_prefix: exclusive prefix over hubs of
+// nb (nb + 1) / 2, nb = ceil(d / 32)) taken from one cursor, read from the
+// hub's word-major slab rows in HBM.
+__global__ __launch_bounds__(256) void hub_mfma4_kernel(const HubMeta* __restrict__ hubs, uint32_t nhubs,
+                                                        const uint32_t* __restrict__ tile_prefix, uint32_t ntiles,
+                                                        uint32_t* __restrict__ cursor,
+                                                        unsigned long long* __restrict__ results,
+                                                        const unsigned long long* __restrict__ slab) {
+    unsigned long long acc = 0;
+    uint32_t qcur = 0xFFFFFFFFu;
+    TaskChunk chunk(ntiles, gridDim.x * (blockDim.x >> 6), 8);
+    while (true) {
+        const uint32_t t = wave_next_task(chunk, cursor);
+        if (t >= ntiles) break;
+        const uint32_t h = wave_bcast(row_of(tile_prefix, nhubs, t));
+        const HubMeta hm = hubs[h];
+        if (hm.q != qcur) {
+            if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
+            qcur = hm.q;
+        }
+        uint32_t I, J;
+        tile_coords(t - tile_prefix[h], &I, &J);
+        acc += mfma4_tile(slab + hm.slab_off, hm.d, hm.d, I, J);
+    }
+    if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
+}
+
 __global__ void reset_u32_kernel(uint32_t* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
@@ -670,6 +707,7 @@ CliqueJob::~CliqueJob() {
     for (auto& h : hub_meta) hipFree(h.second);
     for (auto& h : hub_items) hipFree(h.second);
     for (auto& h : hub_rows) hipFree(h.second);
+    for (auto& h : hub_tiles) hipFree(h.second);
 }
 
 // Must match the carving at the top of clique_block_kernel / hub_rows_kernel.
@@ -788,7 +826,11 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     hipLaunchKernelGGL(hub_rows_kernel<K>, dim3(grid), dim3(block), lds, s, qrefs, meta, hp.nhubs, x.hub_items[K],
                        hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax, hp.wmax, hp.rpc, H,
                        filter_words(hp.dmax));
-    if (K >= 4) {
+    if (K == 4) {
+        const uint32_t cgrid = std::min<uint32_t>((hp.ntiles + 3) / 4, uint32_t(R.num_cus) * 8);
+        hipLaunchKernelGGL(hub_mfma4_kernel, dim3(cgrid), dim3(256), 0, s, meta, hp.nhubs, x.hub_tiles[K], hp.ntiles,
+                           cur + kCursorStride, x.d_result, x.slabs[K]);
+    } else if (K >= 5) {
         const uint32_t cgrid = std::min<uint32_t>((hp.nrows + 3) / 4, uint32_t(R.num_cus) * 8);
         hipLaunchKernelGGL(hub_count_kernel<K>, dim3(cgrid), dim3(256), 0, s, meta, hp.nhubs, x.hub_rows[K], hp.nrows,
                            cur + kCursorStride, x.d_result, x.slabs[K]);
@@ -813,33 +855,41 @@ static int prepare_hubs(CliqueJob& x, int k, const std::vector<uint32_t>& hv, co
     hp.rpc = 64;
     while (hp.rpc > 1 && hub_lds_bytes(k, hp.dmax, hp.wmax, hp.rpc, H) > lds_limit) hp.rpc >>= 1;
     std::vector<HubMeta> meta(hp.nhubs);
-    std::vector<uint32_t> items(hp.nhubs), rows(hp.nhubs);
-    uint64_t slab = 0;
+    std::vector<uint32_t> items(hp.nhubs), rows(hp.nhubs), tiles(hp.nhubs);
+    uint64_t slab = 0, nt = 0;
     uint32_t ni = 0, nr = 0;
     for (uint32_t h = 0; h < hp.nhubs; ++h) {
         const uint32_t d = hd[h];
         const uint32_t W = (d + 63) / 64;
+        const uint64_t nb = (d + 31) / 32;
         meta[h] = HubMeta{hv[h], hq[h], d, W, slab};
         slab += uint64_t(d) * W;
         items[h] = ni;
         rows[h] = nr;
+        tiles[h] = uint32_t(nt);
         ni += (d + hp.rpc - 1) / hp.rpc;
         nr += d;
+        nt += nb * (nb + 1) / 2;
     }
+    if (nt >= (uint64_t(1) << 32)) KZ_FAIL(KZ_ERR_UNSUPPORTED, "%llu hub tiles (> 2^32)", (unsigned long long)nt);
     hp.nitems = ni;
     hp.nrows = nr;
+    hp.ntiles = uint32_t(nt);
     void* dm = nullptr;
-    uint32_t *di = nullptr, *dr = nullptr;
+    uint32_t *di = nullptr, *dr = nullptr, *dt = nullptr;
     KZ_HIP(hipMalloc(&dm, sizeof(HubMeta) * std::max<uint32_t>(1, hp.nhubs)));
     KZ_HIP(hipMalloc(&di, 4 * std::max<uint32_t>(1, hp.nhubs)));
     KZ_HIP(hipMalloc(&dr, 4 * std::max<uint32_t>(1, hp.nhubs)));
-    x.bytes += (sizeof(HubMeta) + 8) * std::max<uint32_t>(1, hp.nhubs);
+    KZ_HIP(hipMalloc(&dt, 4 * std::max<uint32_t>(1, hp.nhubs)));
+    x.bytes += (sizeof(HubMeta) + 12) * std::max<uint32_t>(1, hp.nhubs);
     x.hub_meta[k] = dm;
     x.hub_items[k] = di;
     x.hub_rows[k] = dr;
+    x.hub_tiles[k] = dt;
     KZ_HIP(hipMemcpy(dm, meta.data(), sizeof(HubMeta) * meta.size(), hipMemcpyHostToDevice));
     KZ_HIP(hipMemcpy(di, items.data(), 4 * items.size(), hipMemcpyHostToDevice));
     KZ_HIP(hipMemcpy(dr, rows.data(), 4 * rows.size(), hipMemcpyHostToDevice));
+    KZ_HIP(hipMemcpy(dt, tiles.data(), 4 * tiles.size(), hipMemcpyHostToDevice));
     if (k >= 4) {
         unsigned long long* s = nullptr;
         KZ_HIP(hipMalloc(&s, std::max<uint64_t>(1, slab) * 8));
@@ -950,9 +1000,14 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
     KZ_HIP(hipMalloc(&x.d_result, sizeof(unsigned long long) * std::max<size_t>(1, queries.size())));
     x.bytes += sizeof(QueryRef) * qr.size() + 4 * std::max<size_t>(1, hv.size()) * (queries.size() > 1 ? 2 : 1) +
                4 * size_t(std::max(1, cursor)) + 8 * std::max<size_t>(1, queries.size());
-    // capture
-    Runtime& R = rt();
-    hipStream_t main = R.stream;
+    // capture on a stream of its own: other threads may use the library
+    // stream meanwhile (the server prepares jobs beside running queries)
+    hipStream_t main = nullptr;
+    KZ_HIP(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+    struct StreamFree {
+        hipStream_t s;
+        ~StreamFree() { (void)hipStreamDestroy(s); }
+    } main_free{main};
     const size_t nbr = std::max<size_t>(1, x.groups.size());
     std::vector<hipStream_t> br(nbr, nullptr);
     std::vector<hipEvent_t> join(nbr, nullptr);

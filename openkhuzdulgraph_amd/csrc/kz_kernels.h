@@ -267,12 +267,13 @@ struct CliqueJob {
     // hub roots (last bucket), per k: split into row chunks over many
     // workgroups; rows go to an HBM slab, then one wave counts each row
     struct HubPlan {
-        uint32_t nhubs = 0, dmax = 0, wmax = 0, rpc = 64, nitems = 0, nrows = 0;
+        uint32_t nhubs = 0, dmax = 0, wmax = 0, rpc = 64, nitems = 0, nrows = 0, ntiles = 0;
     };
     std::map<int, HubPlan> hubplan;
     std::map<int, void*> hub_meta;         // HubMeta[nhubs]
     std::map<int, uint32_t*> hub_items;    // exclusive prefix of row chunks per hub
     std::map<int, uint32_t*> hub_rows;     // exclusive prefix of rows per hub
+    std::map<int, uint32_t*> hub_tiles;    // exclusive prefix of 32 x 32 count tiles per hub (k = 4)
     std::map<int, unsigned long long*> slabs;  // per k, HBM bitmap rows
     uint64_t bytes = 0;                        // device memory this job holds
     hipGraph_t graph = nullptr;

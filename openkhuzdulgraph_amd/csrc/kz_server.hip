@@ -1,20 +1,25 @@
-// Serving: a worker thread runs queries on resident graphs (kz_server_*).
+// Serving: queries on resident graphs (kz_server_*).
 //
 // No reference counterpart: the reference runs one app (one pattern, one
 // graph) per MPI job, DistributedApplication::run
 // (src/distributed_application.cc:259-356).  Here many queries share one
-// device:
+// device.  A worker thread owns the admission state; two helper threads do
+// the slow host-side work so that it never holds up admission:
 //   * admission: queries are taken in arrival order; a query needs a
-//     prepared execution (a clique job's task lists and slabs, or the census
-//     copy); the bytes of all prepared executions the server keeps stay
-//     within the budget, idle ones are released least recently used first;
+//     prepared execution (a clique job's task lists and slabs, a symmetric
+//     graph's forward copy, or the census copy); the bytes of every
+//     prepared execution the server keeps (and the estimates of those being
+//     prepared) stay within the budget, idle ones are released cheapest to
+//     rebuild first (GreedyDual over their measured preparation times, so a
+//     large graph's costly job outlives many small ones); a query that must
+//     wait for memory held by running queries holds back the ones behind it;
+//   * preparation (helper thread 1): forward copies and clique jobs are built
+//     while other queries keep being admitted and launched; a query whose
+//     execution is being prepared is passed by later ones;
 //   * continuous batching: an admitted clique query is launched on a free
-//     stream as soon as its execution is ready (up to max_inflight at once),
-//     so new queries start while earlier ones still run; completion is
-//     polled with events and results land in pinned host memory;
-//   * a census query runs on the worker thread itself (its passes are
-//     ordered on the library stream) while launched clique queries proceed
-//     on their streams.
+//     stream as soon as its execution is ready (up to max_inflight at once);
+//     completions are polled with events, results land in pinned memory;
+//   * census queries (helper thread 2) run beside the clique queries.
 
 #include <chrono>
 #include <condition_variable>
@@ -46,13 +51,40 @@ struct SQuery {
     bool done = false;
 };
 
+// Kind of the Prepared entry that accounts for the forward copy of a
+// symmetric graph, which every clique execution on that graph runs on.
+constexpr int kForwardCopy = -1;
+
 // A prepared execution whose device memory the server accounts for.
 struct Prepared {
     kz_graph g = nullptr;
     int kind = 0, k = 0;
-    uint64_t bytes = 0;
-    uint64_t last_use = 0;
-    int busy = 0;  // launched queries using it
+    uint64_t bytes = 0;  // accounted (an estimate while preparing)
+    double cost_ms = 0;  // measured preparation time
+    double H = 0;        // GreedyDual priority: released lowest first
+    int busy = 0;        // launched queries or helper work using it
+    bool preparing = false;
+    bool retried = false;
+};
+
+// Work for a helper thread.
+enum { kJobForward = 1, kJobPrepare = 2, kJobCensus = 3 };
+struct AuxJob {
+    int type = 0;
+    SQuery* q = nullptr;
+    Prepared* p = nullptr;
+    int rc = 0;
+    double ms = 0;
+};
+
+struct AuxWorker {
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<AuxJob> todo;
+    std::vector<AuxJob> done;
+    bool stop = false;
+    int pending = 0;  // submitted, not yet collected (worker thread only)
 };
 
 }  // namespace
@@ -68,6 +100,7 @@ struct kz_server_s {
     std::unordered_map<uint64_t, kz::SQuery*> live;  // submitted, not yet retired
     uint64_t next_ticket = 1;
     bool stopping = false;
+    bool aux_event = false;  // a helper finished a job (guarded by mu)
     // worker-only state
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> events;
@@ -75,7 +108,9 @@ struct kz_server_s {
     std::vector<kz::Prepared*> slot_prep;
     unsigned long long* pinned = nullptr;  // one result per stream slot
     std::vector<kz::Prepared*> prepared;
-    uint64_t in_use = 0, tick = 0;
+    uint64_t in_use = 0;
+    double L = 0;  // GreedyDual inflation
+    kz::AuxWorker prep, census;
 };
 
 namespace kz {
@@ -89,23 +124,30 @@ Prepared* find_prepared(kz_server_s* s, kz_graph g, int kind, int k) {
     return nullptr;
 }
 
-// Kind of the Prepared entry that accounts for the forward copy of a
-// symmetric graph, which every clique execution on that graph runs on.
-constexpr int kForwardCopy = -1;
-
 bool clique_exec_on(const DeviceGraph& d) {
     for (int k = 3; k <= 5; ++k)
         if (d.clique_exec[k]) return true;
     return false;
 }
 
+// A clique execution of graph g is kept or being prepared.  Checked before
+// the graph's own slots are read: a helper may be writing one of them.
+bool clique_prepared_on(kz_server_s* s, kz_graph g) {
+    for (Prepared* p : s->prepared)
+        if (p->g == g && p->kind == KZ_QUERY_CLIQUE) return true;
+    return false;
+}
+
+void touch(kz_server_s* s, Prepared* p) { p->H = s->L + p->cost_ms + 1e-3; }
+
 void release_prepared(kz_server_s* s, Prepared* p) {
     DeviceGraph& d = p->g->g;
-    if (p->kind == KZ_QUERY_CLIQUE) d.clique_exec[p->k].reset();
-    else if (p->kind == kForwardCopy) {
-        // still in use by a clique execution (one being prepared): freed
-        // with it
-        if (!clique_exec_on(d)) release_forward(d);
+    if (p->kind == KZ_QUERY_CLIQUE) {
+        d.clique_exec[p->k].reset();
+    } else if (p->kind == kForwardCopy) {
+        // still used by a clique execution kept outside the server: freed
+        // with the graph
+        if (!clique_prepared_on(s, p->g) && !clique_exec_on(d)) release_forward(d);
     } else {
         d.census.reset();
     }
@@ -116,6 +158,13 @@ void release_prepared(kz_server_s* s, Prepared* p) {
             break;
         }
     delete p;
+}
+
+// An idle execution can be released; a forward copy once no clique
+// execution on its graph is kept or being prepared.
+bool releasable(kz_server_s* s, const Prepared* p) {
+    if (p->busy || p->preparing) return false;
+    return p->kind != kForwardCopy || (!clique_prepared_on(s, p->g) && !clique_exec_on(p->g->g));
 }
 
 // Device bytes the degree orientation of a symmetric graph takes while it
@@ -156,26 +205,42 @@ uint64_t free_device_bytes() {
     return f;
 }
 
-// An idle execution can be released; a forward copy once no clique
-// execution on its graph is left.
-bool releasable(const Prepared* p) {
-    if (p->busy) return false;
-    return p->kind != kForwardCopy || !clique_exec_on(p->g->g);
-}
-
-// Frees idle prepared executions, least recently used first, until `need`
-// more bytes fit in the budget and in the device's free memory.  Returns
-// whether they fit.
+// Frees idle prepared executions, lowest GreedyDual priority first, until
+// `need` more bytes fit in the budget and in the device's free memory.
+// Returns whether they fit.
 bool make_room(kz_server_s* s, uint64_t need) {
     const uint64_t B = budget_of(s);
     while (s->in_use + need > B || (need && free_device_bytes() < need + kFreeMargin)) {
         Prepared* victim = nullptr;
         for (Prepared* p : s->prepared)
-            if (releasable(p) && (!victim || p->last_use < victim->last_use)) victim = p;
+            if (releasable(s, p) && (!victim || p->H < victim->H)) victim = p;
         if (!victim) return false;
+        s->L = victim->H;
         release_prepared(s, victim);
     }
     return true;
+}
+
+// Releases every idle prepared execution except `keep`; returns whether any
+// was released.
+bool release_idle(kz_server_s* s, Prepared* keep) {
+    bool any = false;
+    for (bool again = true; again;) {
+        again = false;
+        for (Prepared* p : s->prepared)
+            if (p != keep && releasable(s, p)) {
+                release_prepared(s, p);
+                any = again = true;
+                break;
+            }
+    }
+    return any;
+}
+
+bool any_busy(const kz_server_s* s) {
+    for (const Prepared* p : s->prepared)
+        if (p->busy || p->preparing) return true;
+    return false;
 }
 
 void finish(kz_server_s* s, SQuery* q, int status) {
@@ -190,182 +255,248 @@ void finish(kz_server_s* s, SQuery* q, int status) {
     s->cv_done.notify_all();
 }
 
-// Releases every idle prepared execution except `keep`; returns whether any
-// was released.
-bool release_idle(kz_server_s* s, Prepared* keep) {
-    bool any = false;
-    for (bool again = true; again;) {
-        again = false;
-        for (Prepared* p : s->prepared)
-            if (p != keep && releasable(p)) {
-                release_prepared(s, p);
-                any = again = true;
-                break;
-            }
+// ------------------------------------------------------- helper threads --
+
+void aux_run(AuxJob& j) {
+    const Clock::time_point t0 = Clock::now();
+    DeviceGraph& d = j.p->g->g;
+    if (j.type == kJobForward) {
+        j.rc = ensure_forward(d);
+    } else if (j.type == kJobPrepare) {
+        CliqueJob* job = nullptr;
+        j.rc = graph_job(d, j.p->k, &job);
+    } else {
+        kz_run_stats st;
+        j.q->t_start = t0;
+        j.rc = motif_census(d, j.q->res.counts, &st);
     }
-    return any;
+    j.ms = ms_between(t0, Clock::now());
 }
 
-bool any_busy(const kz_server_s* s) {
-    for (const Prepared* p : s->prepared)
-        if (p->busy) return true;
-    return false;
+void aux_loop(kz_server_s* s, AuxWorker* w) {
+    (void)hipSetDevice(s->device);
+    while (true) {
+        AuxJob j;
+        {
+            std::unique_lock<std::mutex> lk(w->mu);
+            w->cv.wait(lk, [&] { return w->stop || !w->todo.empty(); });
+            if (w->todo.empty()) return;
+            j = w->todo.front();
+            w->todo.pop_front();
+        }
+        aux_run(j);
+        {
+            std::lock_guard<std::mutex> lk(w->mu);
+            w->done.push_back(j);
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->aux_event = true;
+        }
+        s->cv_work.notify_one();
+    }
 }
 
-// Accounts a fresh preparation of `bytes`; trims idle ones if the estimate
-// was low.  Returns false (and releases it) when it still does not fit.
-bool account(kz_server_s* s, Prepared* p, uint64_t bytes) {
+void aux_submit(AuxWorker& w, const AuxJob& j) {
+    {
+        std::lock_guard<std::mutex> lk(w.mu);
+        w.todo.push_back(j);
+    }
+    w.pending++;
+    w.cv.notify_one();
+}
+
+std::vector<AuxJob> aux_collect(AuxWorker& w) {
+    std::vector<AuxJob> out;
+    {
+        std::lock_guard<std::mutex> lk(w.mu);
+        out.swap(w.done);
+    }
+    w.pending -= int(out.size());
+    return out;
+}
+
+// Registers an execution being prepared (busy, `est` bytes reserved).
+Prepared* reserve(kz_server_s* s, kz_graph g, int kind, int k, uint64_t est) {
+    Prepared* p = new Prepared();
+    p->g = g;
+    p->kind = kind;
+    p->k = k;
+    p->bytes = est;
+    p->preparing = true;
+    p->busy = 1;
+    s->in_use += est;
+    touch(s, p);
+    s->prepared.push_back(p);
+    return p;
+}
+
+// Turns a reservation into the execution's real size and marks it idle.
+// Returns false (and releases it) when it then no longer fits next to the
+// others.
+bool settle(kz_server_s* s, Prepared* p, uint64_t bytes, double ms) {
+    s->in_use -= p->bytes;
     p->bytes = bytes;
     s->in_use += bytes;
-    p->busy++;  // keep it while trimming the others
-    const bool fits = make_room(s, 0);
+    p->cost_ms = ms;
+    p->preparing = false;
+    touch(s, p);
+    const bool fits = make_room(s, 0);  // p is still busy: not a victim
     p->busy--;
     if (!fits) release_prepared(s, p);
     return fits;
 }
 
-// Accounts the forward copy of a symmetric graph (built by graph_job) once
-// per graph.  Returns false when it does not fit the budget; the entry is
-// then dropped.
-bool adopt_forward(kz_server_s* s, kz_graph g) {
-    DeviceGraph& d = g->g;
-    if (d.oriented || !d.fwd_ready || !d.fwd_bytes) return true;
-    if (Prepared* f = find_prepared(s, g, kForwardCopy, 0)) {
-        f->last_use = ++s->tick;
-        return true;
+// Takes the helper threads' finished jobs.  Queries finished here (failed
+// preparations, census results) are appended to *gone.
+void collect_aux(kz_server_s* s, std::vector<SQuery*>* gone) {
+    for (AuxJob& j : aux_collect(s->prep)) {
+        Prepared* p = j.p;
+        const kz_graph g = p->g;
+        DeviceGraph& d = g->g;
+        if (j.rc && (j.rc == KZ_ERR_NOMEM || j.rc == KZ_ERR_DEVICE) && !p->retried && release_idle(s, p)) {
+            // out of device memory despite the estimate: once more, with
+            // every idle execution released
+            p->retried = true;
+            aux_submit(s->prep, j);
+            continue;
+        }
+        if (j.rc) {
+            p->preparing = false;
+            p->busy--;
+            release_prepared(s, p);
+            if (!clique_prepared_on(s, g) && !find_prepared(s, g, kForwardCopy, 0) && d.fwd_ready &&
+                !clique_exec_on(d))
+                release_forward(d);
+            finish(s, j.q, j.rc);
+            gone->push_back(j.q);
+            continue;
+        }
+        const uint64_t bytes =
+            p->kind == kForwardCopy ? d.fwd_bytes : static_cast<CliqueJob*>(d.clique_exec[p->k].get())->bytes;
+        if (!settle(s, p, bytes, j.ms) && !any_busy(s)) {
+            finish(s, j.q, KZ_ERR_BUDGET);
+            gone->push_back(j.q);
+        }
     }
-    Prepared* f = new Prepared{g, kForwardCopy, 0, 0, ++s->tick, 0};
-    s->prepared.push_back(f);
-    return account(s, f, d.fwd_bytes);
+    for (AuxJob& j : aux_collect(s->census)) {
+        Prepared* p = j.p;
+        if (j.rc) {
+            p->busy--;
+            p->preparing = false;
+            if (!census_bytes(p->g->g)) release_prepared(s, p);
+        } else if (p->preparing) {
+            // the first census on this graph: its copy now has a real size
+            settle(s, p, census_bytes(p->g->g), j.ms);
+        } else {
+            p->busy--;
+            touch(s, p);
+        }
+        finish(s, j.q, j.rc);
+    }
 }
 
-// Drops a clique execution that could not be accounted, and the graph's
-// forward copy when nothing uses or accounts for it any more.
-void drop_clique(kz_server_s* s, Prepared* p) {
-    const kz_graph g = p->g;
-    release_prepared(s, p);
-    DeviceGraph& d = g->g;
-    if (d.fwd_ready && !clique_exec_on(d) && !find_prepared(s, g, kForwardCopy, 0)) release_forward(d);
-}
-
-// Prepares (if needed) and launches or runs one query.  Returns false when
-// it must wait for memory held by running queries.
-bool start_query(kz_server_s* s, SQuery* q, int slot) {
+// One admission step for query q.  Returns 1 when q left the pending list
+// (launched, handed to the census thread, or finished), 0 when it stays
+// (being prepared, its execution busy, or no free stream: later queries may
+// pass it), -1 when it must wait for memory (later queries wait too).
+int admit(kz_server_s* s, SQuery* q, int* free_slot) {
     DeviceGraph& d = q->g->g;
     Prepared* p = find_prepared(s, q->g, q->kind, q->k);
-    if (!p) {
-        // an execution prepared before (by a direct call or an earlier
-        // server) is adopted with its real size
-        const bool cached = q->kind == KZ_QUERY_CLIQUE ? bool(d.clique_exec[q->k]) : bool(d.census);
-        if (cached) {
-            p = new Prepared{q->g, q->kind, q->k, 0, 0, 0};
-            s->prepared.push_back(p);
-            const uint64_t bytes =
-                q->kind == KZ_QUERY_CLIQUE ? static_cast<CliqueJob*>(d.clique_exec[q->k].get())->bytes : census_bytes(d);
-            p->busy++;
-            const bool fwd_fits = q->kind != KZ_QUERY_CLIQUE || adopt_forward(s, q->g);
-            p->busy--;
-            if (!fwd_fits) drop_clique(s, p);
-            if (!fwd_fits || !account(s, p, bytes)) {
-                if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
-                return !any_busy(s);
-            }
-        }
-    }
-    if (!p && q->kind == KZ_QUERY_CLIQUE && !d.oriented && !d.fwd_ready) {
-        // a symmetric graph's first clique query: its degree orientation
-        // first (accounted with the graph), then the job, sized from the
-        // exact forward degrees
-        const uint64_t est = forward_estimate(d);
-        if (est > budget_of(s)) {
-            finish(s, q, KZ_ERR_BUDGET);
-            return true;
-        }
-        if (!make_room(s, est)) {
-            if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
-            return !any_busy(s);
-        }
-        int e = ensure_forward(d);
-        if ((e == KZ_ERR_DEVICE || e == KZ_ERR_NOMEM) && release_idle(s, nullptr)) e = ensure_forward(d);
-        if (e) {
-            finish(s, q, e);
-            return true;
-        }
-        if (!adopt_forward(s, q->g)) {
-            release_forward(d);
-            if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
-            return !any_busy(s);
-        }
-    }
-    if (!p) {
-        const uint64_t est = estimate_bytes(d, q->kind, q->k);
-        Prepared* fwd = find_prepared(s, q->g, kForwardCopy, 0);
-        if (fwd) fwd->busy++;  // the job needs it: not a victim for its room
-        const bool room = est <= budget_of(s) && make_room(s, est);
-        if (fwd) fwd->busy--;
-        if (est > budget_of(s)) {
-            finish(s, q, KZ_ERR_BUDGET);
-            return true;
-        }
-        if (!room) {
-            // nothing running would free more: it cannot fit
-            if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
-            return !any_busy(s);
-        }
-        p = new Prepared{q->g, q->kind, q->k, 0, 0, 0};
-        s->prepared.push_back(p);
-        if (q->kind == KZ_QUERY_CLIQUE) {
-            CliqueJob* job = nullptr;
-            int e = graph_job(d, q->k, &job);
-            // out of device memory despite the estimate: release every idle
-            // preparation and try once more
-            if ((e == KZ_ERR_DEVICE || e == KZ_ERR_NOMEM) && release_idle(s, p)) e = graph_job(d, q->k, &job);
-            if (e) {
-                release_prepared(s, p);
-                finish(s, q, e);
-                return true;
-            }
-            // the forward copy the job runs on belongs to the graph and is
-            // accounted once for all its clique executions
-            p->busy++;
-            const bool fwd_fits = adopt_forward(s, q->g);
-            p->busy--;
-            if (!fwd_fits) drop_clique(s, p);
-            if (!fwd_fits || !account(s, p, job->bytes)) {
-                if (!any_busy(s)) finish(s, q, KZ_ERR_BUDGET);
-                return !any_busy(s);
-            }
-        }
-    }
-    p->last_use = ++s->tick;
-    q->t_start = Clock::now();
+    if (p && (p->busy || p->preparing)) return 0;
+    auto no_room = [&]() {
+        if (any_busy(s)) return -1;
+        finish(s, q, KZ_ERR_BUDGET);  // nothing running would free more
+        return 1;
+    };
     if (q->kind == KZ_QUERY_CENSUS) {
-        // runs here, ordered on the library stream
-        kz_run_stats st;
-        const int e = motif_census(d, q->res.counts, &st);
-        if (!e && p->bytes == 0 && !account(s, p, census_bytes(d))) {
-            // it ran, but its state does not fit next to the busy ones: drop it
+        if (s->census.pending) return 0;
+        if (!p) {
+            const bool cached = bool(d.census);
+            const uint64_t est = cached ? census_bytes(d) : estimate_bytes(d, q->kind, 0);
+            if (est > budget_of(s)) {
+                finish(s, q, KZ_ERR_BUDGET);
+                return 1;
+            }
+            if (!make_room(s, est)) return no_room();
+            p = reserve(s, q->g, q->kind, 0, est);
+            // a copy kept from a direct call is adopted with its real size
+            if (cached) p->preparing = false;
+        } else {
+            p->busy++;
         }
-        finish(s, q, e);
-        return true;
+        aux_submit(s->census, AuxJob{kJobCensus, q, p, 0, 0});
+        return 1;
     }
-    CliqueJob* job = nullptr;
-    if (int e = graph_job(d, q->k, &job)) {
-        finish(s, q, e);
-        return true;
+    if (!p) {
+        Prepared* f = nullptr;
+        if (!d.oriented) {
+            f = find_prepared(s, q->g, kForwardCopy, 0);
+            if (f && f->preparing) return 0;
+            if (!f && !d.fwd_ready) {
+                // a symmetric graph's first clique query: its degree
+                // orientation first, then the job, sized from the exact
+                // forward degrees
+                const uint64_t est = forward_estimate(d);
+                if (est > budget_of(s)) {
+                    finish(s, q, KZ_ERR_BUDGET);
+                    return 1;
+                }
+                if (!make_room(s, est)) return no_room();
+                f = reserve(s, q->g, kForwardCopy, 0, est);
+                aux_submit(s->prep, AuxJob{kJobForward, q, f, 0, 0});
+                return 0;
+            }
+            if (!f && d.fwd_bytes) {  // built by a direct call: adopted
+                f = reserve(s, q->g, kForwardCopy, 0, d.fwd_bytes);
+                f->preparing = false;
+                f->busy = 0;
+            }
+        }
+        const bool cached = bool(d.clique_exec[q->k]);
+        const uint64_t est = cached ? static_cast<CliqueJob*>(d.clique_exec[q->k].get())->bytes
+                                    : estimate_bytes(d, q->kind, q->k);
+        if (est > budget_of(s)) {
+            finish(s, q, KZ_ERR_BUDGET);
+            return 1;
+        }
+        if (f) f->busy++;  // the job needs it: not a victim for its room
+        const bool room = make_room(s, est);
+        if (f) f->busy--;
+        if (!room) return no_room();
+        p = reserve(s, q->g, q->kind, q->k, est);
+        if (!cached) {
+            aux_submit(s->prep, AuxJob{kJobPrepare, q, p, 0, 0});
+            return 0;
+        }
+        p->preparing = false;
+        p->busy = 0;
     }
+    // prepared and idle: launch it on a free stream
+    if (*free_slot < 0) return 0;
+    CliqueJob* job = static_cast<CliqueJob*>(d.clique_exec[q->k].get());
+    const int slot = *free_slot;
     p->busy++;
+    touch(s, p);
+    q->t_start = Clock::now();
     s->slot_query[slot] = q;
     s->slot_prep[slot] = p;
-    if (clique_job_launch(*job, s->streams[slot], s->pinned + slot) != KZ_OK ||
+    if (!job || clique_job_launch(*job, s->streams[slot], s->pinned + slot) != KZ_OK ||
         hipEventRecord(s->events[slot], s->streams[slot]) != hipSuccess) {
         (void)hipGetLastError();
         p->busy--;
         s->slot_query[slot] = nullptr;
         s->slot_prep[slot] = nullptr;
         finish(s, q, KZ_ERR_DEVICE);
+        return 1;
     }
-    return true;
+    *free_slot = -1;
+    for (int i = 0; i < int(s->slot_query.size()); ++i)
+        if (!s->slot_query[i]) {
+            *free_slot = i;
+            break;
+        }
+    return 1;
 }
 
 }  // namespace
@@ -377,7 +508,7 @@ static void worker_main(kz_server_s* s) {
     (void)hipSetDevice(s->device);
     const int nslots = int(s->streams.size());
     while (true) {
-        // completions
+        // completions on the streams
         int busy_slots = 0;
         for (int i = 0; i < nslots; ++i) {
             SQuery* q = s->slot_query[i];
@@ -394,48 +525,63 @@ static void worker_main(kz_server_s* s) {
             s->slot_prep[i] = nullptr;
             finish(s, q, e == hipSuccess ? KZ_OK : KZ_ERR_DEVICE);
         }
-        // admission and launch, in arrival order; a query whose execution is
-        // in use by a running query waits for it, later ones may go first
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->aux_event = false;
+        }
+        // the helpers' finished jobs
+        std::vector<SQuery*> gone;
+        collect_aux(s, &gone);
+        const bool helpers_idle = !s->prep.pending && !s->census.pending;
+        // admission and launch, in arrival order
         std::vector<SQuery*> batch;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->pending.empty() && busy_slots == 0) {
+            if (s->pending.empty() && busy_slots == 0 && helpers_idle) {
                 if (s->stopping) break;
-                s->cv_work.wait(lk, [&] { return s->stopping || !s->pending.empty(); });
+                s->cv_work.wait(lk, [&] { return s->stopping || !s->pending.empty() || s->aux_event; });
                 continue;
             }
             batch.assign(s->pending.begin(), s->pending.end());
         }
-        std::vector<SQuery*> started;
-        for (SQuery* q : batch) {
-            Prepared* p = find_prepared(s, q->g, q->kind, q->k);
-            if (p && p->busy) continue;
-            int slot = -1;
-            if (q->kind == KZ_QUERY_CLIQUE) {
-                for (int i = 0; i < nslots; ++i)
-                    if (!s->slot_query[i]) {
-                        slot = i;
-                        break;
-                    }
-                if (slot < 0) break;
+        int free_slot = -1;
+        for (int i = 0; i < nslots; ++i)
+            if (!s->slot_query[i]) {
+                free_slot = i;
+                break;
             }
-            // the memory check must not overtake an earlier waiting query
-            if (!start_query(s, q, slot)) break;
-            started.push_back(q);
-            ++busy_slots;
+        const size_t finished_by_helpers = gone.size();
+        for (SQuery* q : batch) {
+            bool left = false;
+            for (size_t i = 0; i < finished_by_helpers; ++i) left |= gone[i] == q;
+            if (left) continue;
+            const int r = admit(s, q, &free_slot);
+            if (r > 0) gone.push_back(q);
+            if (r < 0) break;  // waiting for memory: later queries do not overtake it
         }
-        if (!started.empty()) {
+        if (!gone.empty()) {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (SQuery* q : started)
+            for (SQuery* q : gone)
                 for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
                     if (*it == q) {
                         s->pending.erase(it);
                         break;
                     }
-        } else if (busy_slots) {
+        } else {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+            if (!s->aux_event) s->cv_work.wait_for(lk, std::chrono::microseconds(50));
         }
+    }
+}
+
+static void stop_helpers(kz_server_s* s) {
+    for (AuxWorker* w : {&s->prep, &s->census}) {
+        {
+            std::lock_guard<std::mutex> lk(w->mu);
+            w->stop = true;
+        }
+        w->cv.notify_all();
+        if (w->th.joinable()) w->th.join();
     }
 }
 
@@ -471,6 +617,8 @@ int kz_server_start(const kz_server_cfg* cfg, kz_server* out) {
     if (hipHostMalloc(reinterpret_cast<void**>(&s->pinned), sizeof(unsigned long long) * n, hipHostMallocDefault) !=
         hipSuccess)
         return fail();
+    s->prep.th = std::thread(aux_loop, s, &s->prep);
+    s->census.th = std::thread(aux_loop, s, &s->census);
     s->worker = std::thread(worker_main, s);
     *out = s;
     return KZ_OK;
@@ -531,11 +679,15 @@ int kz_server_stop(kz_server s) {
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    stop_helpers(s);
     for (auto st : s->streams) (void)hipStreamDestroy(st);
     for (auto ev : s->events) (void)hipEventDestroy(ev);
     (void)hipHostFree(s->pinned);
-    // the executions prepared under this server's budget are released
-    while (!s->prepared.empty()) release_prepared(s, s->prepared.back());
+    // the executions prepared under this server's budget are released,
+    // clique executions before the forward copies they run on
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t i = s->prepared.size(); i-- > 0;)
+            if ((s->prepared[i]->kind == kForwardCopy) == (pass == 1)) release_prepared(s, s->prepared[i]);
     for (auto& kv : s->live) delete kv.second;
     delete s;
     return KZ_OK;

@@ -106,6 +106,29 @@ def test_autotuner_keeps_counts_exact(kzdev):
     dg.release()
 
 
+def _oriented_by_id(kz, g):
+    """The forward-by-id part of a symmetric graph (u -> v for v > u) as an
+    oriented graph, through a .dgraph file (packed 20-byte header, then the
+    sorted {src, dst} pairs): low-id hubs keep their forward degrees."""
+    import tempfile
+
+    import numpy as np
+    rp, col = g.rowptr, g.col
+    src = np.repeat(np.arange(g.num_vertices, dtype=np.uint32), np.diff(rp).astype(np.int64))
+    keep = col > src
+    pairs = np.empty((int(keep.sum()), 2), np.uint32)
+    pairs[:, 0], pairs[:, 1] = src[keep], col[keep]
+    header = np.zeros(1, dtype=[("nv", "<u4"), ("ne", "<u8"), ("nl", "<i4"), ("el", "<i4")])
+    header["nv"], header["ne"] = g.num_vertices, len(pairs)
+    with tempfile.NamedTemporaryFile(suffix=".dgraph") as f:
+        f.write(header.tobytes())
+        f.write(pairs.tobytes())
+        f.flush()
+        o = kz.load_dgraph(f.name)
+    assert o.oriented_flag
+    return o
+
+
 @pytest.mark.parametrize("k", [3, 4, 5])
 def test_hub_and_bucket_paths_exact(kzdev, k):
     """Every bucket path (wave, LDS rows, hub rows in HBM + per-row count) on
@@ -119,7 +142,7 @@ def test_hub_and_bucket_paths_exact(kzdev, k):
     rng = np.random.default_rng(23)
     src = np.concatenate([rng.integers(0, 3000, 60000), np.zeros(2000, np.int64), np.ones(1500, np.int64)])
     dst = np.concatenate([rng.integers(0, 3000, 60000), np.arange(1, 2001), np.arange(2, 1502)])
-    cases.append(("planted_hubs", kz.from_edges(src, dst, 3000)))
+    cases.append(("planted_hubs", _oriented_by_id(kz, kz.from_edges(src, dst, 3000))))
     for name, g in cases:
         print(f"[case] graph={name} k={k} max_degree={g.max_degree}", flush=True)
         want = oracle.run_app(APP[k], g.rowptr, g.col)[1][0]

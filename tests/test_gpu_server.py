@@ -90,3 +90,39 @@ def test_server_budget_eviction_and_rejection(kzdev):
     with kz.Server(hbm_budget=4096, max_inflight=2) as srv:
         for key, res in _run_stream(kz, srv, dgs, keys[:3] + [("chunglu", "census", 0)]):
             assert res["status"] == kz._ffi.KZ_ERR_BUDGET, (key, res)
+
+
+def test_queries_are_admitted_during_a_census_and_a_preparation(kzdev):
+    """A census runs on a helper thread, and a clique job is prepared on
+    another: a small clique query submitted behind either is admitted,
+    launched and finished while they are still running, and every result
+    stays exact."""
+    kz = kzdev
+    big_sym = kz.gen_chung_lu(1500000, 12000000, 2.3, 8000, 7)
+    big_ori = kz.gen_rmat(21, 16, seed=3).oriented()
+    small = kz.gen_rmat(12, 16, seed=4).oriented()
+    want_small = oracle.run_app(APP[4], small.rowptr, small.col)[1][0]
+    want_census = oracle.motif_census(big_sym.rowptr, big_sym.col)
+    want_big = oracle.run_app(APP[3], big_ori.rowptr, big_ori.col)[1][0]
+    d_sym, d_ori, d_small = big_sym.to_device(), big_ori.to_device(), small.to_device()
+    with kz.Server(hbm_budget=0, max_inflight=4) as srv:
+        # the small query's execution is prepared before the timed part
+        assert srv.poll(srv.submit(d_small, "clique", 4), wait=True)["counts"][0] == want_small
+        print("[case] clique query behind a census", flush=True)
+        tc = srv.submit(d_sym, "census", 0)
+        tq = srv.submit(d_small, "clique", 4)
+        rq = srv.poll(tq, wait=True)
+        census_running = srv.poll(tc) is None
+        rc = srv.poll(tc, wait=True)
+        assert rq["status"] == 0 and rq["counts"][0] == want_small
+        assert rc["status"] == 0 and rc["counts"] == want_census
+        assert census_running, (rq, rc)
+        print("[case] clique query behind a preparation", flush=True)
+        tb = srv.submit(d_ori, "clique", 3)
+        tq = srv.submit(d_small, "clique", 4)
+        rq = srv.poll(tq, wait=True)
+        big_running = srv.poll(tb) is None
+        rb = srv.poll(tb, wait=True)
+        assert rq["status"] == 0 and rq["counts"][0] == want_small
+        assert rb["status"] == 0 and rb["counts"][0] == want_big
+        assert big_running, (rq, rb)
