@@ -80,14 +80,15 @@ def cpu_baseline(g, budget_s: float, threads: int) -> dict:
     _, cnt = oracle.run_app("kautomine_four_clique", rp, col, roots=roots, threads=threads)
     dt = time.perf_counter() - t0
     return {
-        "value": b["B4"] / dt / 1e9,
-        "unit": "GB/s (merge-model bytes of the sample / its seconds)",
+        "value": None,  # set by the caller: the sample's time projected to the whole count
+        "unit": "s",
         "cores": threads,
         "kind": "port",
         "sample": f"every {stride}-th root vertex ({len(roots)} of {n}) of the same graph, "
                   f"{cnt[0]} 4-cliques, {b['B4']} merge-model bytes, {dt:.2f} s",
-        "seconds": dt,
+        "sample_seconds": dt,
         "sample_bytes": b["B4"],
+        "sample_merge_model_GBps": b["B4"] / dt / 1e9,
     }
 
 
@@ -595,9 +596,10 @@ def bench_clique(kz, args) -> int:
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         cpu = cpu_baseline(g, args.cpu_seconds, threads)
-        if B4:
-            # the sample's merge-model work scaled to the whole count
-            cpu["projected_full_count_seconds"] = cpu["seconds"] * B4 / cpu["sample_bytes"]
+        # the sample's time scaled to the whole count by merge-model bytes
+        wb = work["B4"] if args.k == 4 else work["B3"]
+        cpu["value"] = cpu["sample_seconds"] * wb / cpu["sample_bytes"] if args.k == 4 else None
+        cpu["projection"] = "sample seconds x (merge-model bytes of the whole count / of the sample)"
         cpu.update(host_cpu())
         log(f"cpu baseline: {cpu}")
 

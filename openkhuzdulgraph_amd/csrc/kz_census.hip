@@ -235,8 +235,10 @@ __global__ __launch_bounds__(256) void census_sums_kernel(const uint64_t* __rest
 // workgroup's n counters and n-entry touched list in HBM (one returning
 // atomic per wedge, the cost of this pass before the LDS counters).
 constexpr uint32_t kC4Slots = 8192;  // 64 KB of LDS (4096: 266.4 ms per census, 8192: 263.7)
+constexpr int kC4Block = 1024;
 
-__global__ __launch_bounds__(256) void census_c4_kernel(const uint64_t* __restrict__ rowptr,
+template <int T>
+__global__ __launch_bounds__(T) void census_c4_kernel(const uint64_t* __restrict__ rowptr,
                                                         const uint32_t* __restrict__ col, uint32_t n,
                                                         uint32_t* __restrict__ cursor, uint32_t* __restrict__ cnt_slab,
                                                         uint32_t* __restrict__ touched_slab,
@@ -395,14 +397,20 @@ static int build_ranked(DeviceGraph& g, Ranked& R) {
     Scratch tmp;
     uint64_t *k0 = nullptr, *k1 = nullptr;
     uint32_t *rank = nullptr, *order = nullptr, *skip = nullptr, *fdeg = nullptr;
+    // the temporaries are freed on every return, failures included
     auto cleanup = [&]() {
-        for (void* p : {(void*)k0, (void*)k1, (void*)rank, (void*)order, (void*)skip, (void*)fdeg, tmp.ptr})
-            if (p) (void)hipFree(p);
+        for (void** p : {(void**)&k0, (void**)&k1, (void**)&rank, (void**)&order, (void**)&skip, (void**)&fdeg,
+                         &tmp.ptr})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
     };
-    auto fail = [&](int e) {
-        cleanup();
-        return e;
-    };
+    struct Cleanup {
+        decltype(cleanup)& f;
+        ~Cleanup() { f(); }
+    } cleanup_on_return{cleanup};
+    auto fail = [&](int e) { return e; };
     const uint64_t kmax = std::max<uint64_t>(n, g.nnz);
     if (hipMalloc(&k0, 8 * std::max<uint64_t>(1, kmax)) != hipSuccess ||
         hipMalloc(&k1, 8 * std::max<uint64_t>(1, kmax)) != hipSuccess ||
@@ -411,7 +419,6 @@ static int build_ranked(DeviceGraph& g, Ranked& R) {
         hipMalloc(&R.rowptr, 8 * (size_t(n) + 1)) != hipSuccess ||
         hipMalloc(&R.col, col_alloc_bytes(g.nnz)) != hipSuccess) {
         (void)hipGetLastError();
-        cleanup();
         KZ_FAIL(KZ_ERR_NOMEM, "census relabelling buffers (%llu entries)", (unsigned long long)g.nnz);
     }
     const uint32_t nb = (n + 255) / 256;
@@ -431,7 +438,6 @@ static int build_ranked(DeviceGraph& g, Ranked& R) {
     if (hipMalloc(&skip, 4 * (size_t(n) + 1)) != hipSuccess || hipMalloc(&fdeg, 4 * (size_t(n) + 1)) != hipSuccess ||
         hipMalloc(&R.frowptr, 8 * (size_t(n) + 1)) != hipSuccess) {
         (void)hipGetLastError();
-        cleanup();
         KZ_FAIL(KZ_ERR_NOMEM, "census forward split");
     }
     KZ_HIP(hipMemsetAsync(fdeg + n, 0, 4, s));
@@ -444,7 +450,6 @@ static int build_ranked(DeviceGraph& g, Ranked& R) {
     R.bytes += col_alloc_bytes(R.fnnz);
     if (hipMalloc(&R.fcol, col_alloc_bytes(R.fnnz)) != hipSuccess) {
         (void)hipGetLastError();
-        cleanup();
         KZ_FAIL(KZ_ERR_NOMEM, "census forward lists");
     }
     if (n)
@@ -452,7 +457,6 @@ static int build_ranked(DeviceGraph& g, Ranked& R) {
                            skip, R.frowptr, R.fcol);
     KZ_HIP(hipGetLastError());
     KZ_HIP(hipStreamSynchronize(s));
-    cleanup();
     return KZ_OK;
 }
 
@@ -563,8 +567,11 @@ static int run_census(Ranked& R, uint64_t* sums, double* device_ms) {
     hipLaunchKernelGGL(census_sums_kernel, dim3(std::max(1, rt_.num_cus * 4)), dim3(256), 0, s, R.frowptr, R.fcol,
                        R.deg, R.n, R.troot, R.tmid, R.sums);
     if (R.n)
-        hipLaunchKernelGGL(census_c4_kernel, dim3(R.c4_grid), dim3(256), 0, s, R.rowptr, R.col, R.n, R.cursor + 2 * kCursorStride,
-                           R.cnt, R.touched, R.sums);
+        // 16 waves per workgroup: the counter slabs bound the workgroups
+        // (two per CU), and the HBM counters' returning atomics need many
+        // waves in flight
+        hipLaunchKernelGGL(census_c4_kernel<kC4Block>, dim3(R.c4_grid), dim3(kC4Block), 0, s, R.rowptr, R.col, R.n,
+                           R.cursor + 2 * kCursorStride, R.cnt, R.touched, R.sums);
     KZ_HIP(hipGetLastError());
     KZ_HIP(hipEventRecord(e1, s));
     unsigned long long h[8] = {0};

@@ -517,6 +517,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
 // matches directly (k = 3).  Dynamic LDS: [row buffer RPC*Wmax u64 (k>=4)]
 // [row descriptors RPC u64][hash keys H u32][L Dmax u32][group prefix
 // RPC+1 u32][filter FW u32][hash vals H u16].
+constexpr uint32_t kHubItemRun = 8;
+
 template <int K>
 __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restrict__ queries,
                                                        const HubMeta* __restrict__ hubs, uint32_t nhubs,
@@ -550,7 +552,9 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
     const uint64_t* rowptr = nullptr;
     const uint32_t* col = nullptr;
     RootSet rs{hk, hv, 0, 0, fb, 0};
-    TaskChunk chunk(nitems, gridDim.x, 8);
+    // items come in runs of at least kHubItemRun: consecutive items are
+    // mostly the same hub's, whose list and hash are staged once per run
+    TaskChunk chunk(nitems, gridDim.x, 8, kHubItemRun);
     while (true) {
         if (threadIdx.x == 0) sItem = thread_next_task(chunk, cursor);
         __syncthreads();

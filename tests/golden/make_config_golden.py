@@ -40,8 +40,9 @@ import oracle  # noqa: E402
 OUT = HERE / "config_counts.json"
 PAT = re.compile(r"Pattern count (\d+): (\d+)")
 
-# pool graphs whose clique counts are recorded (every 256th of the 4096)
-POOL_SUBSET = list(range(0, wl.POOL_SIZE, 256))
+# pool graphs whose clique counts are recorded: every 256th of the 4096
+# (Erdos-Renyi, even) and the RMAT graph after each (odd)
+POOL_SUBSET = sorted(i + j for i in range(0, wl.POOL_SIZE, 256) for j in (0, 1))
 
 
 def graph_entry(g) -> dict:
@@ -119,8 +120,8 @@ def c3(threads: int, reference: bool) -> dict:
 
 
 def c4(threads: int, reference: bool) -> dict:
-    e = {"workload": f"pool graphs {POOL_SUBSET[0]}..{POOL_SUBSET[-1]} step 256 of {wl.POOL_SIZE}, "
-                     "degree-oriented; triangle, 4- and 5-clique", "graphs": {}}
+    e = {"workload": f"pool graphs i and i + 1 for i = 0, 256, ..., {wl.POOL_SIZE - 256} of {wl.POOL_SIZE} "
+                     "(Erdos-Renyi and RMAT), degree-oriented; triangle, 4- and 5-clique", "graphs": {}}
     apps = ("kautomine_triangle", "kautomine_four_clique", "kautomine_five_clique")
     for i in POOL_SUBSET:
         g = wl.pool_graph(kz, i, orient=True)
