@@ -29,6 +29,8 @@ Other workloads (each prints its own JSON line; the default is the one above):
     --workload fsm      labelled frequent subgraph mining (kautomine three_fsm)
                         on the census graph with uniform labels (--labels,
                         --threshold)
+    --workload apps     the unchanged three_motif apps (kautomine, kgraphpi) on
+                        the census graph through the application boundary
     --workload serve    a seeded Poisson stream of queries through kz_server:
                         triangle and 4-clique on a large RMAT graph, 5-clique
                         and motif census on a pool of batch-workload graphs,
@@ -358,6 +360,66 @@ def bench_serve(kz, args) -> int:
     return 0
 
 
+def bench_apps(kz, args) -> int:
+    """The unchanged kautomine and kgraphpi three_motif apps (compiled from
+    the reference sources against the Khuzdul API headers) on the census
+    graph's .dgraph, through the application boundary: probe, plain-form
+    GPU count and the app's own timing; their counts are checked against
+    the motif census of the same graph (kautomine prints the induced wedges
+    and the triangles, kgraphpi the triangles and the non-induced wedges)."""
+    import re
+    import subprocess
+
+    import numpy as np
+
+    from openkhuzdulgraph_amd import workloads as wl
+
+    apps_dir = ROOT / "openkhuzdulgraph_amd" / "khuzdul" / "_apps"
+    if not apps_dir.is_dir():
+        raise SystemExit(f"{apps_dir} missing: the apps are built where the reference sources are")
+    t0 = time.perf_counter()
+    g = wl.chunglu_census(kz)
+    log(f"generated {g} in {time.perf_counter() - t0:.1f} s")
+    kz.init(0)
+    dg = g.to_device()
+    census = dg.motif_census()
+    census_ms = dg.last_stats["device_ms"]
+    dg.release()
+    wedges, tri = census[0], census[1]
+    deg = g.degrees().astype(np.int64)
+    noninduced_wedges = int((deg * (deg - 1) // 2).sum())
+    want = {"kautomine_three_motif": [tri, wedges], "kgraphpi_three_motif": [tri, noninduced_wedges]}
+    results = {}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "chunglu.dgraph")
+        g.store(path)
+        for app, expect in want.items():
+            out = subprocess.run([str(apps_dir / app), path, "1", str(args.steps), "true"], capture_output=True,
+                                 text=True, timeout=1200)
+            if out.returncode != 0:
+                raise SystemExit(f"{app} failed: {out.stderr[-2000:]}")
+            counts = {}
+            for m in re.finditer(r"Pattern count (\d+): (\d+)", out.stdout):
+                counts[int(m.group(1))] = int(m.group(2))
+            got = [counts[i] for i in sorted(counts)]
+            avg = float(re.search(r"Average runtime: ([0-9.]+)", out.stdout).group(1))
+            gpu = [float(x) for x in re.findall(r"GPU runtime: ([0-9.]+)", out.stdout)]
+            results[app] = {"counts": got, "avg_runtime_ms": avg, "gpu_ms_per_run": gpu[-1] if gpu else None,
+                            "equal_to_census": sorted(got) == sorted(expect)}
+            log(f"{app}: {results[app]}")
+            if sorted(got) != sorted(expect):
+                raise SystemExit(f"{app} counts {got} differ from the census' {expect}")
+    line = base_line("3-vertex motif counts through the unchanged three_motif apps: s, 1 MI355X",
+                     results["kautomine_three_motif"]["avg_runtime_ms"] / 1e3, "s", args.steps, 1,
+                     results["kautomine_three_motif"]["avg_runtime_ms"], False,
+                     {"workload": "chunglu_4.85M_43Mdraws_three_motif_apps", "vertices": g.num_vertices,
+                      "edges": g.num_entries // 2})
+    line["apps"] = results
+    line["census_device_ms"] = census_ms
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_gen(kz, args) -> int:
     """RMAT(scale, edge factor) generation + clean-up + degree orientation on
     the host (OpenMP) and on the GPU; both must give the same CSR."""
@@ -442,7 +504,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run known-size copies and read-only sums (calibrates the HBM byte counters) "
                          "and time a 4 GiB copy")
-    ap.add_argument("--workload", choices=["clique", "census", "batch", "serve", "gen", "fsm"], default="clique")
+    ap.add_argument("--workload", choices=["clique", "census", "batch", "serve", "gen", "fsm", "apps"],
+                    default="clique")
     ap.add_argument("--labels", type=int, default=8, help="fsm: number of vertex labels")
     ap.add_argument("--threshold", type=int, default=20000, help="fsm: MNI support threshold")
     ap.add_argument("--host-gen", action="store_true", help="batch/serve: generate on the host, not the GPU")
@@ -474,6 +537,8 @@ def main() -> int:
         return bench_serve(kz, args)
     if args.workload == "gen":
         return bench_gen(kz, args)
+    if args.workload == "apps":
+        return bench_apps(kz, args)
 
     return bench_clique(kz, args)
 

@@ -119,14 +119,23 @@ def c3(threads: int, reference: bool) -> dict:
     return e
 
 
+# 5-cliques are recorded for pool graphs of at most this many draws (the
+# oracle's enumeration of the densest RMAT graphs' 5-cliques takes hours)
+POOL_K5_MAX_DRAWS = 1 << 21
+
+
 def c4(threads: int, reference: bool) -> dict:
     e = {"workload": f"pool graphs i and i + 1 for i = 0, 256, ..., {wl.POOL_SIZE - 256} of {wl.POOL_SIZE} "
-                     "(Erdos-Renyi and RMAT), degree-oriented; triangle, 4- and 5-clique", "graphs": {}}
+                     "(Erdos-Renyi and RMAT), degree-oriented; triangle, 4- and 5-clique (5-cliques of graphs "
+                     f"of at most {POOL_K5_MAX_DRAWS} draws; null otherwise)", "graphs": {}}
     apps = ("kautomine_triangle", "kautomine_four_clique", "kautomine_five_clique")
     for i in POOL_SUBSET:
         g = wl.pool_graph(kz, i, orient=True)
+        kind, a, b, _ = wl.pool_params(i)
+        draws = b if kind == "er" else b << a
         t0 = time.perf_counter()
-        cnt = [oracle.run_app(a, g.rowptr, g.col, threads=threads)[1][0] for a in apps]
+        cnt = [oracle.run_app(app, g.rowptr, g.col, threads=threads)[1][0]
+               if app != apps[2] or draws <= POOL_K5_MAX_DRAWS else None for app in apps]
         e["graphs"][str(i)] = {"params": list(wl.pool_params(i)), **graph_entry(g), "k345": cnt,
                                "oracle_seconds": round(time.perf_counter() - t0, 1)}
         print(f"  pool {i}: {cnt}", flush=True)

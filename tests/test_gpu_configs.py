@@ -109,7 +109,8 @@ def test_c4_pool_subset_batched_equals_the_oracle(kzdev):
     got = b.run()
     b.release()
     for r, i in enumerate(idx):
-        assert [int(x) for x in got[r]] == e["graphs"][str(i)]["k345"], i
+        want = e["graphs"][str(i)]["k345"]
+        assert [int(x) for x, w in zip(got[r], want) if w is not None] == [w for w in want if w is not None], i
     for dg in dgs:
         dg.release()
 
