@@ -817,13 +817,19 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     }
     // hubs: rows kernel, then (k >= 4) the count kernel on the same stream
     const CliqueJob::HubPlan& hp = x.hubplan[K];
-    const uint32_t block = x.cfg.block[gr.bucket];
+    uint32_t block = x.cfg.block[gr.bucket];
     uint32_t H, hshift;
     hash_size(hp.dmax, &H, &hshift);
     (void)hshift;
     const size_t lds = hub_lds_bytes(K, hp.dmax, hp.wmax, hp.rpc, H);
     if (lds > lds_limit) KZ_FAIL(KZ_ERR_UNSUPPORTED, "hub rows need %zu bytes of LDS (limit %zu)", lds, lds_limit);
     uint32_t per_cu = std::max<uint32_t>(1, uint32_t(R.lds_per_block / (lds + kStaticLdsReserve)));
+    // 1024-thread workgroups of the hub rows kernel (80 VGPRs: 24 waves per
+    // CU) run one per CU; where the LDS leaves room for three 512-thread
+    // ones, those fill the CU instead (RMAT-24 4-cliques 211.2 -> 205.3 ms;
+    // where it does not, as for the forward degrees of RMAT-26/ef32, 1024
+    // threads stay faster: 6.42 against 7.95 s)
+    if (block == 1024 && per_cu >= 3) block = 512;
     per_cu = std::min<uint32_t>(per_cu, 2048 / block);
     const uint32_t grid = std::min<uint32_t>(hp.nitems, uint32_t(R.num_cus) * per_cu);
     const HubMeta* meta = static_cast<const HubMeta*>(x.hub_meta[K]);
