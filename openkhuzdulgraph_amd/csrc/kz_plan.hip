@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 
 #include "kz_device.h"
@@ -41,7 +42,8 @@ uint64_t g_level_budget = uint64_t(4) << 30;  // memory_size_per_level default (
 
 
 constexpr int kWavesPerBlock = 4;
-constexpr uint32_t kBatch = 32;  // embeddings per cursor grab (siblings stay together)
+constexpr uint32_t kBatch = 64;  // embeddings per cursor grab: one per lane (siblings stay together)
+constexpr int kFlatUnroll = 4;   // stream chunks loaded together by the flat path
 
 struct DevLevel {
     uint64_t n = 0;
@@ -83,6 +85,16 @@ struct LevelArgs {
     uint32_t temp_cap;
 };
 
+// Per-wave LDS of the flat path: each sibling lane's stream offset, list
+// start and new vertex (kEmpty when it repeats an ancestor's vertex).
+struct FlatLds {
+    uint64_t start[64];
+    uint32_t off[64];
+    uint32_t w[64];
+    uint64_t anc_idx[KZ_MAX_PATTERN];  // the segment's ancestors: indexes at sizes 1.. and vertices
+    uint32_t anc_m[KZ_MAX_PATTERN];
+};
+
 __device__ __forceinline__ void add_agg(unsigned long long* acc, int agg, unsigned long long v) {
     switch (agg) {
         case 0: acc[0] += v; break;
@@ -96,156 +108,489 @@ __device__ __forceinline__ void add_agg(unsigned long long* acc, int agg, unsign
     }
 }
 
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t lane) {
+    const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
+    const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), lane);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+template <int N>
+__device__ __forceinline__ void sort_small(uint32_t* v) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j + 1 < N - i; ++j) {
+            const uint32_t lo = min(v[j], v[j + 1]), hi = max(v[j], v[j + 1]);
+            v[j] = lo;
+            v[j + 1] = hi;
+        }
+}
+
+// Position class of x against sorted matched values z[0..N) that may
+// repeat: 2 * (distinct values below x) + (x equals one of them).
+template <int N>
+__device__ __forceinline__ uint32_t pos_class_sorted(uint32_t x, const uint32_t* z) {
+    uint32_t lt = 0;
+    bool eq = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const bool distinct = i == 0 || z[i] != z[i - 1];
+        lt += distinct && z[i] < x;
+        eq |= z[i] == x;
+    }
+    return 2 * lt + eq;
+}
+
+// The same for the ancestors' sorted values plus one more vertex w (kEmpty
+// when w repeats an ancestor: it then adds no class boundary).
+template <int N>
+__device__ __forceinline__ uint32_t pos_class_plus(uint32_t x, const uint32_t* z, uint32_t w) {
+    const uint32_t c = pos_class_sorted<N>(x, z);
+    return c + 2 * (w < x) + (w == x && !(c & 1u));
+}
+
+// Number of elements of the sorted list A whose position class against the
+// sorted values z[0..N) is in pred: each class is a run of A between two
+// binary-searched bounds (src/vertex_set.cc:20-86 restated as counts).
+template <int N>
+__device__ uint64_t count_classes(const uint32_t* A, uint32_t n, const uint32_t* z, uint32_t pred) {
+    uint64_t c = 0;
+    uint32_t from = 0, t = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if (i > 0 && z[i] == z[i - 1]) continue;
+        const uint32_t lo = from + lower_bound_global(A + from, n - from, z[i]);
+        const uint32_t hi = lo + upper_bound_global(A + lo, n - lo, z[i]);
+        if ((pred >> (2 * t)) & 1u) c += lo - from;
+        if ((pred >> (2 * t + 1)) & 1u) c += hi - lo;
+        from = hi;
+        ++t;
+    }
+    if ((pred >> (2 * t)) & 1u) c += n - from;
+    return c;
+}
+
+__device__ __forceinline__ bool hash_has(const uint32_t* hk, uint32_t hmask, uint32_t hshift, uint32_t x) {
+    uint32_t h = (x * 2654435761u) >> hshift;
+    while (true) {
+        const uint32_t key = hk[h];
+        if (key == x) return true;
+        if (key == kEmpty) return false;
+        h = (h + 1) & hmask;
+    }
+}
+
+// Rebuild embedding e of size S: matched vertices m[0..S) and ancestor
+// indexes idx[1..S] (idx[S] = e) by walking parents (src/engine.cc:1008-1021).
+template <int S>
+__device__ __forceinline__ void rebuild(const LevelArgs& a, uint64_t e, uint32_t* m, uint64_t* idx) {
+    idx[S] = e;
+    uint64_t cur = e;
+#pragma unroll
+    for (int t = S; t >= 2; --t) {
+        m[t - 1] = a.lv[t].vertex[cur];
+        cur = a.lv[t].parent[cur];
+        idx[t - 1] = cur;
+    }
+    m[0] = a.lv[1].root_base + uint32_t(cur);
+}
+
+// One embedding extended by the whole wave: the general path for every
+// step list.
+__device__ void one_embedding(const LevelArgs& a, uint64_t e, uint32_t* hk, HashCache& hc, unsigned long long* acc,
+                              uint32_t* temp) {
+    const uint32_t lane = threadIdx.x & 63;
+    const int s = a.s;
+    // rebuild the embedding (src/engine.cc:1008-1021)
+    uint32_t m[KZ_MAX_PATTERN];
+    uint64_t idx[KZ_MAX_PATTERN + 1];
+    idx[s] = e;
+    uint64_t cur = e;
+    for (int t = s; t >= 2; --t) {
+        m[t - 1] = a.lv[t].vertex[cur];
+        cur = a.lv[t].parent[cur];
+        idx[t - 1] = cur;
+    }
+    m[0] = a.lv[1].root_base + uint32_t(cur);
+    const uint8_t li = a.types[order_code(m, s)];
+    if (li == 0xFF) {
+        if (lane == 0) atomicOr(a.err, 1u);
+        return;
+    }
+    const kz_step_list& L = a.lists[li];
+    const uint32_t fuse = a.fuse[li];
+    // sorted distinct matched values for position classes
+    uint32_t z[KZ_MAX_PATTERN];
+    int q = 0;
+    for (int i = 0; i < s; ++i) {
+        bool dup = false;
+        for (int j = 0; j < q; ++j) dup |= z[j] == m[i];
+        if (!dup) z[q++] = m[i];
+    }
+    for (int i = 1; i < q; ++i)
+        for (int j = i; j > 0 && z[j - 1] > z[j]; --j) {
+            const uint32_t t = z[j];
+            z[j] = z[j - 1];
+            z[j - 1] = t;
+        }
+    uint32_t rlen[KZ_MAX_OPS] = {0, 0, 0, 0};
+    auto resolve = [&](const kz_set_ref& r) -> SetView {
+        SetView v;
+        if (r.kind == KZ_SET_NBR) {
+            const uint32_t x = m[r.a];
+            const uint64_t b = a.rowptr[x];
+            v.p = a.col + b;
+            v.n = uint32_t(a.rowptr[x + 1] - b);
+        } else if (r.a == s) {
+            v.p = temp + uint64_t(r.b) * a.temp_cap;
+            v.n = rlen[r.b];
+        } else {
+            const DevLevel& Lv = a.lv[r.a];
+            const uint64_t i = idx[r.a];
+            v.p = Lv.slot_data[r.b] + (Lv.slot_off[r.b][i] - Lv.slot_base[r.b]);
+            v.n = Lv.slot_len[r.b][i];
+        }
+        return v;
+    };
+    // an operand is shared by siblings when it comes from an ancestor
+    auto shared = [&](const kz_set_ref& r) {
+        return (r.kind == KZ_SET_NBR && int(r.a) < s - 1) || (r.kind == KZ_SET_SLOT && int(r.a) < s);
+    };
+    for (int j = 0; j < L.num_ops; ++j) {
+        const kz_op& op = L.ops[j];
+        if (fuse & (1u << j)) {
+            // result only feeds COUNT outputs: count while streaming
+            if (a.mode != 0) continue;
+            unsigned long long c[KZ_MAX_OUTS] = {0, 0, 0, 0};
+            wave_set_op(op.op, resolve(op.lhs), resolve(op.rhs), shared(op.lhs), shared(op.rhs), hk, hc,
+                        [&](uint32_t x, bool keep, uint64_t, uint32_t) {
+                            const uint32_t cls = keep ? pos_class(x, z, q) : 0;
+                            for (int o = 0; o < L.num_outs; ++o) {
+                                const kz_out& out = L.outs[o];
+                                if (out.src.kind != KZ_SET_SLOT || out.src.a != s || out.src.b != j) continue;
+                                c[o] += __popcll(__ballot(keep && ((out.pred >> cls) & 1u)));
+                            }
+                        });
+            for (int o = 0; o < L.num_outs; ++o) {
+                const kz_out& out = L.outs[o];
+                if (out.src.kind == KZ_SET_SLOT && out.src.a == s && out.src.b == j)
+                    add_agg(acc, out.aggregator, c[o]);
+            }
+            continue;
+        }
+        uint32_t* outp = temp + uint64_t(j) * a.temp_cap;
+        rlen[j] = wave_set_op(op.op, resolve(op.lhs), resolve(op.rhs), shared(op.lhs), shared(op.rhs), hk, hc,
+                              [&](uint32_t x, bool keep, uint64_t msk, uint32_t k) {
+                                  if (keep) outp[k + __popcll(msk & lanemask_lt())] = x;
+                              });
+    }
+    uint32_t nch = 0;
+    uint64_t cbase = 0;
+    if (a.mode == 1) cbase = a.child_off[e - a.begin] - a.child_base;
+    for (int o = 0; o < L.num_outs; ++o) {
+        const kz_out& out = L.outs[o];
+        if (out.src.kind == KZ_SET_SLOT && out.src.a == s && (fuse & (1u << out.src.b))) continue;
+        if (out.kind == KZ_OUT_COUNT && a.mode != 0) continue;
+        const SetView src = resolve(out.src);
+        uint32_t cnt = 0;
+        for (uint32_t base = 0; base < src.n; base += 64) {
+            const uint32_t i = base + lane;
+            bool pass = false;
+            uint32_t v = 0;
+            if (i < src.n) {
+                v = src.p[i];
+                pass = (out.pred >> pos_class(v, z, q)) & 1u;
+            }
+            const uint64_t msk = __ballot(pass);
+            if (out.kind == KZ_OUT_SCATTER && a.mode == 1 && pass) {
+                const uint64_t at = cbase + nch + cnt + __popcll(msk & lanemask_lt());
+                a.child_parent[at] = uint32_t(e);
+                a.child_vertex[at] = v;
+            }
+            cnt += __popcll(msk);
+        }
+        if (out.kind == KZ_OUT_COUNT) add_agg(acc, out.aggregator, cnt);
+        else nch += cnt;
+    }
+    if (a.mode == 0) {
+        if (lane == 0) {
+            if (a.nchild) a.nchild[e - a.begin] = nch;
+            for (int j = 0; j < KZ_MAX_OPS; ++j)
+                if (a.cached_mask & (1u << j)) a.slot_len_out[j][e - a.begin] = j < L.num_ops ? rlen[j] : 0;
+        }
+    } else {
+        for (int j = 0; j < L.num_ops; ++j) {
+            if (!(a.cached_mask & (1u << j))) continue;
+            uint32_t* dst = a.slot_data_out[j] + (a.slot_off_out[j][e - a.begin] - a.slot_base_out[j]);
+            const uint32_t* srcp = temp + uint64_t(j) * a.temp_cap;
+            for (uint32_t i = lane; i < rlen[j]; i += 64) dst[i] = srcp[i];
+        }
+    }
+}
+
+// Flat path (pass A only).  Lanes [b, en) of the wave hold sibling
+// embeddings -- children of one parent -- that take the same step list, and
+// that list only counts: every op intersects or subtracts an ancestor's set
+// P (the same for all ops) and the new vertex's list N(w), and every output
+// is a COUNT of an op's result, of N(w) or of an ancestor's set.  The
+// siblings' lists N(w) are concatenated and streamed by all 64 lanes at
+// once (each lane finds its element's sibling by a binary search of the
+// offsets), probing P's LDS hash once per element; counts of ancestor sets
+// (and the P part of P \ N(w)) are class runs found by binary search.  The
+// counts equal the per-embedding path's: the outputs only add up.
+template <int S>
+__device__ void flat_segment(const LevelArgs& a, const kz_step_list& L, uint32_t b, uint32_t en, const uint32_t* m,
+                             const uint64_t* idx, uint64_t wstart, uint32_t wlen, uint32_t* hk, HashCache& hc,
+                             FlatLds& fl, unsigned long long* acc, unsigned long long* lacc) {
+    static_assert(S >= 2, "the flat path needs an ancestor");
+    const uint32_t lane = threadIdx.x & 63;
+    const bool sib = lane >= b && lane < en;
+    // ancestors (the same on every lane of the segment), published by its
+    // first lane for operand lookups by plan index
+    wave_lds_sync();
+    if (lane == b) {
+#pragma unroll
+        for (int t = 0; t < S - 1; ++t) fl.anc_m[t] = m[t];
+#pragma unroll
+        for (int t = 1; t < S; ++t) fl.anc_idx[t] = idx[t];
+    }
+    wave_lds_sync();
+    uint32_t ms[S - 1];
+#pragma unroll
+    for (int t = 0; t < S - 1; ++t) ms[t] = fl.anc_m[t];
+    auto resolve_shared = [&](const kz_set_ref& r) -> SetView {
+        SetView v;
+        if (r.kind == KZ_SET_NBR) {
+            const uint32_t x = fl.anc_m[r.a];
+            const uint64_t o = a.rowptr[x];
+            v.p = a.col + o;
+            v.n = uint32_t(a.rowptr[x + 1] - o);
+        } else {
+            const DevLevel& Lv = a.lv[r.a];
+            const uint64_t i = fl.anc_idx[r.a];
+            v.p = Lv.slot_data[r.b] + (Lv.slot_off[r.b][i] - Lv.slot_base[r.b]);
+            v.n = Lv.slot_len[r.b][i];
+        }
+        return v;
+    };
+    auto varying = [](const kz_set_ref& r) { return r.kind == KZ_SET_NBR && int(r.a) == S - 1; };
+    uint32_t zs[S - 1];
+#pragma unroll
+    for (int t = 0; t < S - 1; ++t) zs[t] = ms[t];
+    sort_small<S - 1>(zs);
+    const uint32_t w = m[S - 1];
+    bool wdup = false;
+#pragma unroll
+    for (int t = 0; t < S - 1; ++t) wdup |= zs[t] == w;
+    uint32_t zw[S];  // this lane's sibling: ancestors' values and w, sorted
+#pragma unroll
+    for (int t = 0; t < S - 1; ++t) zw[t] = zs[t];
+    zw[S - 1] = w;
+    sort_small<S>(zw);
+    // per output: 0 = count an ancestor's set per sibling, 1 = stream and
+    // keep P's members, 2 = stream and keep non-members, 3 = stream all;
+    // bit 2: subtract the streamed count from a per-sibling count of P
+    const bool has_ops = L.num_ops > 0;
+    SetView P{nullptr, 0};
+    if (has_ops) {
+        const kz_op& o0 = L.ops[0];
+        P = resolve_shared(varying(o0.lhs) ? o0.rhs : o0.lhs);
+    }
+    uint32_t omode[KZ_MAX_OUTS] = {0, 0, 0, 0};
+    bool need_stream = false;
+#pragma unroll
+    for (int o = 0; o < KZ_MAX_OUTS; ++o) {
+        if (o >= L.num_outs) break;
+        const kz_out& out = L.outs[o];
+        uint32_t md = 0;
+        if (out.src.kind == KZ_SET_SLOT && out.src.a == S) {
+            const kz_op& op = L.ops[out.src.b];
+            if (op.op == KZ_OP_INTERSECT) md = 1;
+            else if (varying(op.lhs)) md = 2;
+            else md = 1 | 4;
+        } else if (varying(out.src)) {
+            md = 3;
+        }
+        omode[o] = md;
+        need_stream |= (md & 3) != 0;
+    }
+    if (need_stream) {
+        const uint64_t start = sib ? wstart : 0;
+        const uint32_t len = sib ? wlen : 0;
+        uint32_t incl = len;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(incl, d);
+            if (lane >= uint32_t(d)) incl += t;
+        }
+        const uint32_t T = __builtin_amdgcn_readlane(incl, 63);
+        wave_lds_sync();
+        fl.off[lane] = incl - len;
+        fl.start[lane] = start;
+        fl.w[lane] = wdup ? kEmpty : w;
+        // P's hash: at least 4 slots per key where the table allows (short
+        // probe runs); only this path fills the table in the flat kernels
+        uint32_t hmask = 0, hshift = 0;
+        const bool probe = has_ops && P.n > 0;
+        const bool hashed = probe && P.n <= kHashCap / 2;
+        if (hashed) {
+            hash_geometry(P.n <= kHashCap / 4 ? 2 * P.n : P.n, &hmask, &hshift);
+            if (!(hc.p == P.p && hc.n == P.n)) {
+                for (uint32_t i = lane; i <= hmask; i += 64) hk[i] = kEmpty;
+                wave_lds_sync();
+                for (uint32_t i = lane; i < P.n; i += 64) {
+                    const uint32_t x = P.p[i];
+                    uint32_t h = (x * 2654435761u) >> hshift;
+                    while (atomicCAS(&hk[h], kEmpty, x) != kEmpty) h = (h + 1) & hmask;
+                }
+                hc.p = P.p;
+                hc.n = P.n;
+            }
+        }
+        wave_lds_sync();
+        // per output, the position classes counted for a member of P and for
+        // a non-member (0 for outputs not streamed)
+        uint32_t cls_in[KZ_MAX_OUTS], cls_out[KZ_MAX_OUTS];
+#pragma unroll
+        for (int o = 0; o < KZ_MAX_OUTS; ++o) {
+            const uint32_t md = omode[o] & 3u;
+            const uint32_t pred = o < L.num_outs ? L.outs[o].pred : 0u;
+            cls_in[o] = (md == 1 || md == 3) ? pred : 0u;
+            cls_out[o] = (md == 2 || md == 3) ? pred : 0u;
+        }
+        uint32_t c[KZ_MAX_OUTS] = {0, 0, 0, 0};  // this lane's counts
+        // kFlatUnroll chunks of 64 positions per pass, all their list loads in
+        // flight before the first probe.  Branch-free: positions past the end
+        // are clamped to the last one and not counted.
+        uint32_t lo = b;  // sibling of this lane's previous position (positions only grow)
+        for (uint32_t base = 0; base < T; base += 64 * kFlatUnroll) {
+            uint32_t xs[kFlatUnroll], ws[kFlatUnroll];
+#pragma unroll
+            for (int k = 0; k < kFlatUnroll; ++k) {
+                const uint32_t p = min(base + 64 * k + lane, T - 1);
+                // the last sibling whose offset is <= p holds p (its list is
+                // non-empty); siblings lie in [lo, en), en - lo <= 64
+#pragma unroll
+                for (uint32_t step = 32; step; step >>= 1) {
+                    const uint32_t t = lo + step;
+                    const uint32_t ot = fl.off[min(t, en - 1)];
+                    lo = (t < en && ot <= p) ? t : lo;
+                }
+                xs[k] = a.col[fl.start[lo] + (p - fl.off[lo])];
+                ws[k] = fl.w[lo];
+            }
+#pragma unroll
+            for (int k = 0; k < kFlatUnroll; ++k) {
+                const bool valid = base + 64 * k + lane < T;
+                bool found = false;
+                if (probe) found = hashed ? hash_has(hk, hmask, hshift, xs[k]) : bsearch_global(P.p, P.n, xs[k]);
+                const uint32_t cls = pos_class_plus<S - 1>(xs[k], zs, ws[k]);
+#pragma unroll
+                for (int o = 0; o < KZ_MAX_OUTS; ++o)
+                    c[o] += valid && (((found ? cls_in[o] : cls_out[o]) >> cls) & 1u);
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < KZ_MAX_OUTS; ++o) {
+            if (!(omode[o] & 3u)) continue;
+            add_agg(lacc, L.outs[o].aggregator, (omode[o] & 4u) ? 0ull - c[o] : (unsigned long long)c[o]);
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < KZ_MAX_OUTS; ++o) {
+        if (o >= L.num_outs) break;
+        if ((omode[o] & 3u) && !(omode[o] & 4u)) continue;
+        const kz_out& out = L.outs[o];
+        const SetView A = (omode[o] & 4u) ? P : resolve_shared(out.src);
+        if (sib) add_agg(lacc, out.aggregator, count_classes<S>(A.p, A.n, zw, out.pred));
+    }
+}
+
+// FLAT: every list at size S takes the flat path (pass A); else every
+// embedding takes the general path.  (One kernel holding both paths needs
+// twice the registers of either.)
+template <int S, bool FLAT>
 __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
     __shared__ uint32_t sHash[kWavesPerBlock][kHashCap];
+    __shared__ FlatLds sFlat[FLAT ? kWavesPerBlock : 1];
+    static_assert(!FLAT || S >= 2, "the flat path needs an ancestor");
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t w = threadIdx.x >> 6;
     uint32_t* hk = sHash[w];
     const uint64_t gwave = uint64_t(blockIdx.x) * kWavesPerBlock + w;
     uint32_t* temp = a.temp + gwave * uint64_t(KZ_MAX_OPS) * a.temp_cap;
-    const int s = a.s;
     HashCache hc{nullptr, 0};
-    unsigned long long acc[KZ_MAX_AGGREGATORS];
+    unsigned long long acc[KZ_MAX_AGGREGATORS];   // wave-uniform counts
+    unsigned long long lacc[KZ_MAX_AGGREGATORS];  // per-lane counts (flat path)
 #pragma unroll
-    for (int i = 0; i < KZ_MAX_AGGREGATORS; ++i) acc[i] = 0;
+    for (int i = 0; i < KZ_MAX_AGGREGATORS; ++i) acc[i] = lacc[i] = 0;
     while (true) {
         uint64_t first = 0;
         if (lane == 0) first = atomicAdd(reinterpret_cast<unsigned long long*>(a.cursor), (unsigned long long)kBatch);
-        first = __shfl(first, 0) + a.begin;
+        // (register allocation: readlane suits the flat form, shfl the general one)
+        if constexpr (FLAT) first = readlane_u64(first, 0) + a.begin;
+        else first = __shfl(first, 0) + a.begin;
         if (first >= a.end) break;
         const uint64_t last = first + kBatch < a.end ? first + kBatch : a.end;
-        for (uint64_t e = first; e < last; ++e) {
-            // rebuild the embedding (src/engine.cc:1008-1021)
-            uint32_t m[KZ_MAX_PATTERN];
-            uint64_t idx[KZ_MAX_PATTERN + 1];
-            idx[s] = e;
-            uint64_t cur = e;
-            for (int t = s; t >= 2; --t) {
-                m[t - 1] = a.lv[t].vertex[cur];
-                cur = a.lv[t].parent[cur];
-                idx[t - 1] = cur;
-            }
-            m[0] = a.lv[1].root_base + uint32_t(cur);
-            const uint32_t code = order_code(m, s);
-            const uint8_t li = a.types[code];
-            if (li == 0xFF) {
-                if (lane == 0) atomicOr(a.err, 1u);
+        if constexpr (FLAT) {
+            {
+                // one embedding per lane; runs of lanes with the same parent and
+                // step list are segments
+                const uint64_t e = first + lane;
+                const bool valid = e < last;
+                uint32_t m[S];
+                uint64_t idx[S + 1];
+                uint32_t li = 0xFF;
+                uint64_t parent = ~0ull;
+                uint64_t wstart = 0;  // the new vertex's list
+                uint32_t wlen = 0;
+                if (valid) {
+                    rebuild<S>(a, e, m, idx);
+                    li = a.types[order_code(m, S)];
+                    parent = idx[S - 1];
+                    wstart = a.rowptr[m[S - 1]];
+                    wlen = uint32_t(a.rowptr[m[S - 1] + 1] - wstart);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < S; ++t) m[t] = 0;
+#pragma unroll
+                    for (int t = 0; t <= S; ++t) idx[t] = 0;
+                }
+                const uint64_t pp = __shfl_up(parent, 1);
+                const uint32_t pl = __shfl_up(li, 1);
+                uint64_t heads = __ballot(valid && (lane == 0 || pp != parent || pl != li));
+                const uint32_t nvalid = uint32_t(last - first);
+                while (heads) {
+                    const uint32_t b = uint32_t(__builtin_ctzll(heads));
+                    heads &= heads - 1;
+                    const uint32_t en = heads ? uint32_t(__builtin_ctzll(heads)) : nvalid;
+                    const uint32_t sli = __builtin_amdgcn_readlane(li, b);
+                    if (sli == 0xFF) {
+                        if (lane == 0) atomicOr(a.err, 1u);
+                        continue;
+                    }
+                    flat_segment<S>(a, a.lists[sli], b, en, m, idx, wstart, wlen, hk, hc, sFlat[w], acc, lacc);
+                    // flat lists make no children and cache no prefix
+                    if (lane >= b && lane < en) {
+                        if (a.nchild) a.nchild[e - a.begin] = 0;
+#pragma unroll
+                        for (int j = 0; j < KZ_MAX_OPS; ++j)
+                            if (a.cached_mask & (1u << j)) a.slot_len_out[j][e - a.begin] = 0;
+                    }
+                }
                 continue;
             }
-            const kz_step_list& L = a.lists[li];
-            const uint32_t fuse = a.fuse[li];
-            // sorted distinct matched values for position classes
-            uint32_t z[KZ_MAX_PATTERN];
-            int q = 0;
-            for (int i = 0; i < s; ++i) {
-                bool dup = false;
-                for (int j = 0; j < q; ++j) dup |= z[j] == m[i];
-                if (!dup) z[q++] = m[i];
-            }
-            for (int i = 1; i < q; ++i)
-                for (int j = i; j > 0 && z[j - 1] > z[j]; --j) {
-                    const uint32_t t = z[j];
-                    z[j] = z[j - 1];
-                    z[j - 1] = t;
-                }
-            uint32_t rlen[KZ_MAX_OPS] = {0, 0, 0, 0};
-            auto resolve = [&](const kz_set_ref& r) -> SetView {
-                SetView v;
-                if (r.kind == KZ_SET_NBR) {
-                    const uint32_t x = m[r.a];
-                    const uint64_t b = a.rowptr[x];
-                    v.p = a.col + b;
-                    v.n = uint32_t(a.rowptr[x + 1] - b);
-                } else if (r.a == s) {
-                    v.p = temp + uint64_t(r.b) * a.temp_cap;
-                    v.n = rlen[r.b];
-                } else {
-                    const DevLevel& Lv = a.lv[r.a];
-                    const uint64_t i = idx[r.a];
-                    v.p = Lv.slot_data[r.b] + (Lv.slot_off[r.b][i] - Lv.slot_base[r.b]);
-                    v.n = Lv.slot_len[r.b][i];
-                }
-                return v;
-            };
-            // an operand is shared by siblings when it comes from an ancestor
-            auto shared = [&](const kz_set_ref& r) {
-                return (r.kind == KZ_SET_NBR && int(r.a) < s - 1) || (r.kind == KZ_SET_SLOT && int(r.a) < s);
-            };
-            for (int j = 0; j < L.num_ops; ++j) {
-                const kz_op& op = L.ops[j];
-                if (fuse & (1u << j)) {
-                    // result only feeds COUNT outputs: count while streaming
-                    if (a.mode != 0) continue;
-                    unsigned long long c[KZ_MAX_OUTS] = {0, 0, 0, 0};
-                    wave_set_op(op.op, resolve(op.lhs), resolve(op.rhs), shared(op.lhs), shared(op.rhs), hk, hc,
-                                [&](uint32_t x, bool keep, uint64_t, uint32_t) {
-                                    const uint32_t cls = keep ? pos_class(x, z, q) : 0;
-                                    for (int o = 0; o < L.num_outs; ++o) {
-                                        const kz_out& out = L.outs[o];
-                                        if (out.src.kind != KZ_SET_SLOT || out.src.a != s || out.src.b != j) continue;
-                                        c[o] += __popcll(__ballot(keep && ((out.pred >> cls) & 1u)));
-                                    }
-                                });
-                    for (int o = 0; o < L.num_outs; ++o) {
-                        const kz_out& out = L.outs[o];
-                        if (out.src.kind == KZ_SET_SLOT && out.src.a == s && out.src.b == j)
-                            add_agg(acc, out.aggregator, c[o]);
-                    }
-                    continue;
-                }
-                uint32_t* outp = temp + uint64_t(j) * a.temp_cap;
-                rlen[j] = wave_set_op(op.op, resolve(op.lhs), resolve(op.rhs), shared(op.lhs), shared(op.rhs), hk, hc,
-                                      [&](uint32_t x, bool keep, uint64_t msk, uint32_t k) {
-                                          if (keep) outp[k + __popcll(msk & lanemask_lt())] = x;
-                                      });
-            }
-            uint32_t nch = 0;
-            uint64_t cbase = 0;
-            if (a.mode == 1) cbase = a.child_off[e - a.begin] - a.child_base;
-            for (int o = 0; o < L.num_outs; ++o) {
-                const kz_out& out = L.outs[o];
-                if (out.src.kind == KZ_SET_SLOT && out.src.a == s && (fuse & (1u << out.src.b))) continue;
-                if (out.kind == KZ_OUT_COUNT && a.mode != 0) continue;
-                const SetView src = resolve(out.src);
-                uint32_t cnt = 0;
-                for (uint32_t base = 0; base < src.n; base += 64) {
-                    const uint32_t i = base + lane;
-                    bool pass = false;
-                    uint32_t v = 0;
-                    if (i < src.n) {
-                        v = src.p[i];
-                        pass = (out.pred >> pos_class(v, z, q)) & 1u;
-                    }
-                    const uint64_t msk = __ballot(pass);
-                    if (out.kind == KZ_OUT_SCATTER && a.mode == 1 && pass) {
-                        const uint64_t at = cbase + nch + cnt + __popcll(msk & lanemask_lt());
-                        a.child_parent[at] = uint32_t(e);
-                        a.child_vertex[at] = v;
-                    }
-                    cnt += __popcll(msk);
-                }
-                if (out.kind == KZ_OUT_COUNT) add_agg(acc, out.aggregator, cnt);
-                else nch += cnt;
-            }
-            if (a.mode == 0) {
-                if (lane == 0) {
-                    if (a.nchild) a.nchild[e - a.begin] = nch;
-                    for (int j = 0; j < KZ_MAX_OPS; ++j)
-                        if (a.cached_mask & (1u << j)) a.slot_len_out[j][e - a.begin] = j < L.num_ops ? rlen[j] : 0;
-                }
-            } else {
-                for (int j = 0; j < L.num_ops; ++j) {
-                    if (!(a.cached_mask & (1u << j))) continue;
-                    uint32_t* dst = a.slot_data_out[j] + (a.slot_off_out[j][e - a.begin] - a.slot_base_out[j]);
-                    const uint32_t* srcp = temp + uint64_t(j) * a.temp_cap;
-                    for (uint32_t i = lane; i < rlen[j]; i += 64) dst[i] = srcp[i];
-                }
-            }
         }
+        if constexpr (!FLAT)
+            for (uint64_t e = first; e < last; ++e) one_embedding(a, e, hk, hc, acc, temp);
     }
-    if (lane == 0 && a.mode == 0) {
-        for (int i = 0; i < KZ_MAX_AGGREGATORS; ++i)
-            if (acc[i]) atomicAdd(&a.agg[i], acc[i]);
+    if (a.mode == 0) {
+#pragma unroll
+        for (int i = 0; i < KZ_MAX_AGGREGATORS; ++i) {
+            unsigned long long v = acc[i];
+            if constexpr (FLAT) v += wave_sum_u64(lacc[i]);
+            if (lane == 0 && v) atomicAdd(&a.agg[i], v);
+        }
     }
 }
 
@@ -298,6 +643,46 @@ static int scan_u32(const uint32_t* in, uint64_t* out, uint64_t n, Scratch& tmp,
 
 // ----------------------------------------------------------------- host --
 
+// Whether step list L at size s can take the flat path: every op is
+// counted while streaming (not cached, read by no later op, feeding COUNT
+// outputs only), intersects or subtracts one ancestor set P -- the same for
+// every op -- and the new vertex's list N(m[s-1]); every output is a COUNT
+// of an op's result, of N(m[s-1]) or of an ancestor set.
+static bool flat_eligible(const kz_step_list& L, int s, uint32_t cached) {
+    auto varying = [&](const kz_set_ref& r) { return r.kind == KZ_SET_NBR && int(r.a) == s - 1; };
+    auto shared = [&](const kz_set_ref& r) {
+        return (r.kind == KZ_SET_NBR && int(r.a) < s - 1) || (r.kind == KZ_SET_SLOT && int(r.a) < s);
+    };
+    auto same = [](const kz_set_ref& x, const kz_set_ref& y) { return x.kind == y.kind && x.a == y.a && x.b == y.b; };
+    auto is_op = [&](const kz_set_ref& r, int j) { return r.kind == KZ_SET_SLOT && int(r.a) == s && int(r.b) == j; };
+    kz_set_ref P{};
+    for (int j = 0; j < L.num_ops; ++j) {
+        const kz_op& op = L.ops[j];
+        if (cached & (1u << j)) return false;
+        for (int k = j + 1; k < L.num_ops; ++k)
+            if (is_op(L.ops[k].lhs, j) || is_op(L.ops[k].rhs, j)) return false;
+        bool read = false;
+        for (int o = 0; o < L.num_outs; ++o)
+            if (is_op(L.outs[o].src, j)) read = true;
+        if (!read) return false;
+        kz_set_ref sh;
+        if (varying(op.lhs) && shared(op.rhs)) sh = op.rhs;
+        else if (shared(op.lhs) && varying(op.rhs)) sh = op.lhs;
+        else return false;
+        if (op.op != KZ_OP_INTERSECT && op.op != KZ_OP_SUBTRACT) return false;
+        if (j == 0) P = sh;
+        else if (!same(P, sh)) return false;
+    }
+    for (int o = 0; o < L.num_outs; ++o) {
+        const kz_out& out = L.outs[o];
+        if (out.kind != KZ_OUT_COUNT) return false;
+        const kz_set_ref& r = out.src;
+        const bool op_result = r.kind == KZ_SET_SLOT && int(r.a) == s && int(r.b) < L.num_ops;
+        if (!op_result && !varying(r) && !shared(r)) return false;
+    }
+    return true;
+}
+
 struct LevelScratch {
     Scratch nchild, child_off, slot_len[KZ_MAX_OPS], slot_off[KZ_MAX_OPS], cost, bounds;
     Scratch child_parent, child_vertex, slot_data[KZ_MAX_OPS];
@@ -316,6 +701,7 @@ struct Executor {
     LevelScratch ls[KZ_MAX_PATTERN + 1];
     uint32_t cached_mask[KZ_MAX_PATTERN + 1] = {};
     bool has_scatter[KZ_MAX_PATTERN + 1] = {};
+    bool flat_level[KZ_MAX_PATTERN + 1] = {};  // every list at this size takes the flat path
     uint32_t grid = 0;
     uint32_t temp_cap = 0;
     kz_run_stats* st = nullptr;
@@ -400,6 +786,18 @@ struct Executor {
         }
         KZ_HIP(hipMalloc(&d_fuse, fuse.size()));
         KZ_HIP(hipMemcpy(d_fuse, fuse.data(), fuse.size(), hipMemcpyHostToDevice));
+        // KZ_PLAN_FLAT=0 keeps every size on the general path (for comparisons);
+        // stream offsets are 32-bit: 64 lists of fewer than 2^26 entries
+        const char* env = std::getenv("KZ_PLAN_FLAT");
+        const bool flat_on = !(env && env[0] == '0') && g.max_degree < (1u << 26);
+        for (int s = 2; s < K && flat_on; ++s) {
+            bool all = true;
+            for (int c = 0; c < 256; ++c) {
+                const uint8_t li = p.type_table[s][c];
+                if (li != 0xFF) all &= flat_eligible(p.lists[li], s, cached_mask[s]);
+            }
+            flat_level[s] = all;
+        }
         KZ_HIP(hipMalloc(&d_lists, sizeof(kz_step_list) * std::max<uint32_t>(1, p.num_step_lists)));
         KZ_HIP(hipMalloc(&d_types, sizeof(p.type_table)));
         KZ_HIP(hipMalloc(&d_cursor, 8));
@@ -429,7 +827,26 @@ struct Executor {
         la.col = g.col;
         la.temp = static_cast<uint32_t*>(temp.ptr);
         la.temp_cap = temp_cap;
-        hipLaunchKernelGGL(plan_level_kernel, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, la);
+        // a flat size only counts: its pass A is the whole level
+        const bool f = flat_level[la.s];
+        if (f && la.mode != 0) return KZ_OK;
+        const dim3 gd(grid), bd(64 * kWavesPerBlock);
+        switch (la.s) {
+            case 1: hipLaunchKernelGGL((plan_level_kernel<1, false>), gd, bd, 0, s, la); break;
+            case 2:
+                if (f) hipLaunchKernelGGL((plan_level_kernel<2, true>), gd, bd, 0, s, la);
+                else hipLaunchKernelGGL((plan_level_kernel<2, false>), gd, bd, 0, s, la);
+                break;
+            case 3:
+                if (f) hipLaunchKernelGGL((plan_level_kernel<3, true>), gd, bd, 0, s, la);
+                else hipLaunchKernelGGL((plan_level_kernel<3, false>), gd, bd, 0, s, la);
+                break;
+            case 4:
+                if (f) hipLaunchKernelGGL((plan_level_kernel<4, true>), gd, bd, 0, s, la);
+                else hipLaunchKernelGGL((plan_level_kernel<4, false>), gd, bd, 0, s, la);
+                break;
+            default: KZ_FAIL(KZ_ERR_INVALID, "embedding size %d outside 1..%d", la.s, KZ_MAX_PATTERN - 1);
+        }
         KZ_HIP(hipGetLastError());
         if (st) st->launches++;
         return KZ_OK;

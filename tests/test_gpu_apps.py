@@ -51,10 +51,16 @@ def test_unchanged_apps_print_reference_counts(method):
 
 
 @pytest.mark.skipif(not _apps(), reason="app binaries were not built from the reference sources")
-def test_plan_executor_matches_oracle_on_probed_plans(kzdev, tmp_path):
+@pytest.mark.parametrize("flat", ["1", "0"])
+def test_plan_executor_matches_oracle_on_probed_plans(kzdev, tmp_path, monkeypatch, flat):
+    """Both paths of the executor's counting sizes: sibling runs streamed
+    as one (flat, the default) and one embedding per wave (KZ_PLAN_FLAT=0).
+    The Chung-Lu graph with exponent 2.0 has 14 vertices of degree above
+    1,024, whose lists are probed by binary search instead of the LDS hash."""
     kz = kzdev
+    monkeypatch.setenv("KZ_PLAN_FLAT", flat)
     graphs = [kz.gen_rmat(11, 16, seed=8), kz.gen_chung_lu(3000, 20000, 2.3, 300, 4),
-              kz.gen_rmat(11, 16, seed=8).oriented()]
+              kz.gen_rmat(11, 16, seed=8).oriented(), kz.gen_chung_lu(20000, 150000, 2.0, 4000, 11)]
     for app in sorted(oracle.APPS):
         dump = tmp_path / f"{app}.plans"
         env = dict(os.environ, KZ_PROBE_ONLY="1", KZ_PLAN_DUMP=str(dump))
@@ -66,7 +72,7 @@ def test_plan_executor_matches_oracle_on_probed_plans(kzdev, tmp_path):
         for g in graphs:
             dg = g.to_device()
             for p in plans:
-                print(f"[case] app={app} graph={g}", flush=True)
+                print(f"[case] app={app} graph={g} flat={flat}", flush=True)
                 want = oracle.run_plan(p, g.rowptr, g.col)
                 assert dg.count_plan(p) == want, (app, str(g))
             dg.release()
