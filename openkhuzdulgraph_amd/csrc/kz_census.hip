@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -46,6 +47,7 @@ namespace kz {
 namespace {
 
 constexpr uint32_t kSmallRoot = 64;  // forward degree handled by one wave
+constexpr int kCursors = 6;          // triangles (small, large roots), 4-cycles (wave, block, heavy)
 
 struct U64Cast {
     __host__ __device__ uint64_t operator()(uint32_t x) const { return x; }
@@ -65,16 +67,21 @@ struct Ranked {
     uint32_t nsmall = 0, nlarge = 0, large_dmax = 0;
     uint32_t* troot = nullptr;  // per forward edge (u, v): triangles with u lowest
     uint32_t* tmid = nullptr;   // per forward edge: triangles with it as (v, w)
-    uint32_t c4_grid = 0;
-    uint32_t* cnt = nullptr;      // c4_grid * n counters
-    uint32_t* touched = nullptr;  // c4_grid * n touched lists
-    uint32_t* cursor = nullptr;   // 4 task cursors, kCursorStride words apart
+    uint64_t* c4_bound = nullptr;  // per vertex: the 4-cycle pass's wedge bound b(u)
+    uint32_t* c4_tasks = nullptr;  // roots of the wave, block and heavy 4-cycle kernels, in that order
+    uint32_t n_c4_wave = 0, n_c4_block = 0, n_c4_heavy = 0;
+    uint32_t c4_heavy_grid = 0;
+    uint32_t c4_scratch_keys = 0;    // endpoints per heavy workgroup
+    uint32_t c4_max_slots = 0;       // bucket hash slots used (a power of two <= kC4HeavySlots)
+    uint32_t c4_direct_bound = 0;    // heavy roots up to this bound first try one unbucketed pass
+    uint32_t* c4_scratch = nullptr;  // c4_heavy_grid * c4_scratch_keys endpoints
+    uint32_t* cursor = nullptr;      // kCursors task cursors, kCursorStride words apart
     unsigned long long* sums = nullptr;  // 8 accumulators
     uint64_t bytes = 0;           // device memory held (with the 4-clique job)
     CliqueJob k4;
     ~Ranked() {
         for (void* p : {(void*)rowptr, (void*)col, (void*)deg, (void*)frowptr, (void*)fcol, (void*)tasks, (void*)troot,
-                        (void*)tmid, (void*)cnt, (void*)touched, (void*)cursor, (void*)sums})
+                        (void*)tmid, (void*)c4_bound, (void*)c4_tasks, (void*)c4_scratch, (void*)cursor, (void*)sums})
             if (p) (void)hipFree(p);
     }
 };
@@ -229,119 +236,491 @@ __global__ __launch_bounds__(256) void census_sums_kernel(const uint64_t* __rest
     }
 }
 
-// Pass 3: 4-cycles.  Tasks are all vertices, highest rank first.  A root
-// whose wedges (bounded by the degrees of its neighbours ranked below it)
-// fit an LDS hash of kC4Slots counters counts there; the others use the
-// workgroup's n counters and n-entry touched list in HBM (one returning
-// atomic per wedge, the cost of this pass before the LDS counters).
-constexpr uint32_t kC4Slots = 8192;  // 64 KB of LDS (4096: 266.4 ms per census, 8192: 263.7)
+// Pass 3: 4-cycles, each counted once from its highest-ranked vertex u as
+// the pairs of wedges u - v - w, u - x - w (v, w, x ranked below u) with the
+// same endpoint w: sum over w of C(c(w), 2), c(w) = the wedges ending at w.
+// Roots are split by their wedge bound b(u) = sum of d(v) over v in N(u)
+// below u (c4_bound_kernel): b <= kC4WaveBound one wave per root with a
+// per-wave LDS hash of the endpoints; b <= kC4BlockBound one workgroup per
+// root with a workgroup hash; above it the workgroup buckets the root's
+// endpoints by a hash (an LDS histogram, then a scatter into an HBM scratch
+// region, bucket after bucket) and counts each bucket in the LDS hash.
+constexpr uint32_t kC4WaveSlots = 1024;    // per-wave hash: keys and counts, 8 KB
+constexpr uint32_t kC4WaveBound = kC4WaveSlots / 2;
+constexpr uint32_t kC4Slots = 8192;        // workgroup hash of the block kernel, 64 KB
+constexpr uint32_t kC4BlockBound = kC4Slots / 2;
 constexpr int kC4Block = 1024;
+constexpr uint32_t kC4HeavySlots = 16384;  // bucket hash of the heavy kernel, 128 KB
+constexpr uint32_t kC4MaxBuckets = 4096;
+constexpr uint32_t kC4BucketKeys = 4096;   // mean endpoints per bucket aimed at
+constexpr uint32_t kC4ScratchKeys = 12u << 20;  // per heavy workgroup: 48 MB of endpoints
+constexpr uint32_t kC4BucketMul = 0x85EBCA6Bu;  // bucket hash (the tables use 2654435761)
+constexpr int kC4Unroll = 4;                    // wedge stream loads in flight per lane
+constexpr uint32_t kC4DirectBound = kC4HeavySlots / 2;  // heavy roots counted in one unbucketed pass
+constexpr uint32_t kC4MaxProbe = 64;            // bucket table probe limit
+constexpr uint32_t kAllBuckets = 0xFFFFFFFFu;
 
-template <int T>
-__global__ __launch_bounds__(T) void census_c4_kernel(const uint64_t* __restrict__ rowptr,
-                                                        const uint32_t* __restrict__ col, uint32_t n,
-                                                        uint32_t* __restrict__ cursor, uint32_t* __restrict__ cnt_slab,
-                                                        uint32_t* __restrict__ touched_slab,
-                                                        unsigned long long* __restrict__ sums) {
-    __shared__ uint32_t sTask, sTouched, sBound;
+// b(u) for every vertex (one wave per vertex).
+__global__ void c4_bound_kernel(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                const uint32_t* __restrict__ deg, uint32_t n, uint64_t* __restrict__ bound) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wpb = blockDim.x >> 6;
+    for (uint32_t u = blockIdx.x * wpb + (threadIdx.x >> 6); u < n; u += gridDim.x * wpb) {
+        const uint64_t ub = rowptr[u];
+        const uint32_t du = uint32_t(rowptr[u + 1] - ub);
+        unsigned long long sum = 0;
+        for (uint32_t base = 0; base < du; base += 64) {
+            const uint32_t i = base + lane;
+            const uint32_t v = i < du ? col[ub + i] : 0xFFFFFFFFu;
+            if (v < u) sum += deg[v];
+            if (__ballot(v >= u)) break;  // sorted lists: the rest ranks above u
+        }
+        sum = wave_sum_u64(sum);
+        if (lane == 0) bound[u] = sum;
+    }
+}
+
+__device__ __forceinline__ uint32_t shfl_u32(uint32_t v, uint32_t l) { return uint32_t(__shfl(int(v), int(l))); }
+
+// The wedges u - v - w (v = N(u)[k], k < pre; w in N(v), w < u) of one root,
+// streamed by one wave: groups of 64 neighbours v (k0, k0 + kstep, ...),
+// whose lists are concatenated; each lane walks its own contiguous range of
+// ceil(T / 64) positions of the concatenation (found once per group by a
+// binary search of the list offsets), kC4Unroll loads in flight, moving to
+// the next list when it reaches its offset.  The neighbours are in rank
+// order, i.e. by degree, so a group's lists have similar lengths.  f(w) runs
+// on the lanes holding an endpoint.  `wl` is the wave's LDS: 65 offsets
+// (the last one the total) and 64 list starts (census lists hold fewer
+// than 2^31 entries).
+template <class F>
+__device__ __forceinline__ void c4_wave_wedges(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                               uint64_t ub, uint32_t pre, uint32_t u, uint32_t k0, uint32_t kstep,
+                                               uint32_t* wl, F&& f) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t* soff = wl;
+    uint32_t* sstart = wl + 65;
+    for (uint32_t k = k0; k < pre; k += kstep) {
+        uint32_t vb = 0, dv = 0;
+        if (k + lane < pre) {
+            const uint32_t v = ldg(col, ub + k + lane);
+            const uint64_t b = ldg(rowptr, v);
+            vb = uint32_t(b);
+            dv = uint32_t(ldg(rowptr, v + 1) - b);
+        }
+        uint32_t incl = dv;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = uint32_t(__shfl_up(int(incl), d));
+            if (lane >= uint32_t(d)) incl += t;
+        }
+        const uint32_t T = wave_lane(incl, 63);
+        wave_lds_sync();
+        soff[lane] = incl - dv;
+        sstart[lane] = vb;
+        if (lane == 63) soff[64] = T;
+        wave_lds_sync();
+        const uint32_t q = (T + 63) / 64;
+        uint32_t p = min(lane * q, T);
+        const uint32_t e = min(p + q, T);
+        // the list holding p: the last whose offset is <= p (non-empty)
+        uint32_t lo = 0;
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1) {
+            const uint32_t t = lo + step;
+            if (t < 64 && soff[t] <= p) lo = t;
+        }
+        uint32_t nxt = soff[lo + 1];
+        uint32_t adr = sstart[lo] - soff[lo];  // col index of position p is adr + p (mod 2^32)
+        while (p < e) {
+            uint32_t ws[kC4Unroll];
+            const uint32_t cnt = min(uint32_t(kC4Unroll), e - p);
+#pragma unroll
+            for (int c = 0; c < kC4Unroll; ++c) {
+                ws[c] = 0xFFFFFFFFu;
+                if (uint32_t(c) < cnt) {
+                    if (p == nxt) {
+                        do {
+                            ++lo;
+                            nxt = soff[lo + 1];
+                        } while (nxt == p);
+                        adr = sstart[lo] - soff[lo];
+                    }
+                    ws[c] = ldg(col, uint64_t(adr + p));
+                    ++p;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kC4Unroll; ++c)
+                if (ws[c] < u) f(ws[c]);
+        }
+    }
+}
+
+// Counts w into an open-addressing LDS hash of (key, count) with H slots.
+__device__ __forceinline__ void c4_hash_add(uint32_t* key, uint32_t* cnt, uint32_t hmask, uint32_t hshift, uint32_t w) {
+    uint32_t h = hash_slot(w, hshift);
+    while (true) {
+        const uint32_t k = key[h];
+        if (k == w) break;
+        if (k == kEmpty) {
+            const uint32_t old = atomicCAS(&key[h], kEmpty, w);
+            if (old == kEmpty || old == w) break;
+        }
+        h = (h + 1) & hmask;
+    }
+    atomicAdd(&cnt[h], 1u);
+}
+
+__device__ __forceinline__ void c4_geometry(uint32_t keys, uint32_t* H, uint32_t* hshift) {
+    uint32_t h = 64, bits = 6;
+    while (h < 2 * keys) {
+        h <<= 1;
+        ++bits;
+    }
+    *H = h;
+    *hshift = 32 - bits;
+}
+
+// Roots with b(u) <= kC4WaveBound: one wave each.
+__global__ __launch_bounds__(256) void census_c4_wave_kernel(const uint64_t* __restrict__ rowptr,
+                                                             const uint32_t* __restrict__ col,
+                                                             const uint32_t* __restrict__ tasks,
+                                                             const uint64_t* __restrict__ bound, uint32_t ntasks,
+                                                             uint32_t* __restrict__ cursor,
+                                                             unsigned long long* __restrict__ sums) {
+    __shared__ uint32_t sKey[4][kC4WaveSlots], sCnt[4][kC4WaveSlots];
+    __shared__ uint32_t sList[4][129];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t* key = sKey[wave];
+    uint32_t* cnt = sCnt[wave];
+    for (uint32_t i = lane; i < kC4WaveSlots; i += 64) {
+        key[i] = kEmpty;
+        cnt[i] = 0;
+    }
+    wave_lds_sync();
+    unsigned long long acc = 0;
+    TaskChunk chunk(ntasks, gridDim.x * 4, 8);
+    while (true) {
+        const uint32_t t = wave_next_task(chunk, cursor);
+        if (t >= ntasks) break;
+        const uint32_t u = wave_bcast(tasks[t]);
+        const uint64_t ub = ldg(rowptr, u);
+        const uint32_t du = uint32_t(ldg(rowptr, u + 1) - ub);
+        const uint32_t pre = lower_bound_global(col + ub, du, u);
+        uint32_t H, hshift;
+        c4_geometry(uint32_t(bound[u]), &H, &hshift);
+        // the table is clear: every root clears what it used while summing
+        c4_wave_wedges(rowptr, col, ub, pre, u, 0, 64, sList[wave],
+                       [&](uint32_t w) { c4_hash_add(key, cnt, H - 1, hshift, w); });
+        wave_lds_sync();
+        for (uint32_t i = lane; i < H; i += 64) {
+            const uint64_t c = cnt[i];
+            acc += c * (c - 1) / 2;
+            key[i] = kEmpty;
+            cnt[i] = 0;
+        }
+        wave_lds_sync();
+    }
+    const unsigned long long x = wave_sum_u64(acc);
+    if (lane == 0 && x) atomicAdd(&sums[6], x);
+}
+
+// Roots with kC4WaveBound < b(u) <= kC4BlockBound: one workgroup each.
+__global__ __launch_bounds__(kC4Block) void census_c4_block_kernel(const uint64_t* __restrict__ rowptr,
+                                                                   const uint32_t* __restrict__ col,
+                                                                   const uint32_t* __restrict__ tasks,
+                                                                   const uint64_t* __restrict__ bound,
+                                                                   uint32_t ntasks, uint32_t* __restrict__ cursor,
+                                                                   unsigned long long* __restrict__ sums) {
+    __shared__ uint32_t sTask;
     __shared__ uint32_t sKey[kC4Slots], sCnt[kC4Slots];
-    uint32_t* cnt = cnt_slab + size_t(blockIdx.x) * n;
-    uint32_t* touched = touched_slab + size_t(blockIdx.x) * n;
+    __shared__ uint32_t sList[kC4Block / 64][129];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nw = blockDim.x >> 6;
     unsigned long long acc = 0;
-    if (threadIdx.x == 0) sTouched = 0;
-    TaskChunk chunk(n, gridDim.x, 8);
+    for (uint32_t i = threadIdx.x; i < kC4Slots; i += blockDim.x) {
+        sKey[i] = kEmpty;
+        sCnt[i] = 0;
+    }
+    TaskChunk chunk(ntasks, gridDim.x, 8);
     while (true) {
-        if (threadIdx.x == 0) {
-            sTask = thread_next_task(chunk, cursor);
-            sBound = 0;
-        }
+        if (threadIdx.x == 0) sTask = thread_next_task(chunk, cursor);
         __syncthreads();
         const uint32_t t = wave_bcast(sTask);
-        if (t >= n) break;
-        const uint32_t u = n - 1 - t;
+        if (t >= ntasks) break;
+        const uint32_t u = wave_bcast(tasks[t]);
         const uint64_t ub = ldg(rowptr, u);
         const uint32_t du = uint32_t(ldg(rowptr, u + 1) - ub);
-        const uint32_t pre = lower_bound_global(col + ub, du, u);  // neighbours ranked below u
-        // wedge bound: the degrees of the neighbours below u
-        // (64-bit sums, each wave's clamped to 2^16 before the LDS add, so
-        // that no sum wraps and a root above the LDS size never takes it)
-        unsigned long long part = 0;
-        for (uint32_t k = threadIdx.x; k < pre; k += blockDim.x) {
-            const uint32_t v = ldg(col, ub + k);
-            part += ldg(rowptr, v + 1) - ldg(rowptr, v);
-        }
-        part = wave_sum_u64(part);
-        if (lane == 0 && part) atomicAdd(&sBound, uint32_t(part < 65536ull ? part : 65536ull));
+        const uint32_t pre = lower_bound_global(col + ub, du, u);
+        uint32_t H, hshift;
+        c4_geometry(uint32_t(bound[u]), &H, &hshift);
+        // the table is clear: every root clears what it used while summing
+        c4_wave_wedges(rowptr, col, ub, pre, u, wave * 64, nw * 64, sList[wave],
+                       [&](uint32_t w) { c4_hash_add(sKey, sCnt, H - 1, hshift, w); });
         __syncthreads();
-        const uint32_t bound = wave_bcast(sBound);
-        if (bound <= kC4Slots / 2) {
-            // LDS counters: an open-addressing hash of at least twice the
-            // wedges, so every probe ends
-            uint32_t H = 64;
-            while (H < 2 * bound) H <<= 1;
-            uint32_t bits = 0;
-            while ((1u << bits) < H) ++bits;
-            const uint32_t hshift = 32 - bits;
-            for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) {
-                sKey[i] = kEmpty;
-                sCnt[i] = 0;
-            }
-            __syncthreads();
-            for (uint32_t k = wave; k < pre; k += nw) {
-                const uint32_t v = ldg(col, ub + k);
-                const uint64_t vb = ldg(rowptr, v);
-                const uint32_t dv = uint32_t(ldg(rowptr, v + 1) - vb);
-                for (uint32_t base = 0; base < dv; base += 64) {
-                    const uint32_t i = base + lane;
-                    const uint32_t w = i < dv ? ldg(col, vb + i) : 0xFFFFFFFFu;
-                    if (w < u) {
+        for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) {
+            const uint64_t c = sCnt[i];
+            acc += c * (c - 1) / 2;
+            sKey[i] = kEmpty;
+            sCnt[i] = 0;
+        }
+        __syncthreads();
+    }
+    const unsigned long long x = wave_sum_u64(acc);
+    if (lane == 0 && x) atomicAdd(&sums[6], x);
+}
+
+// Exclusive scan of a[0..P) in place by the whole workgroup; a[P] = total.
+__device__ void block_exclusive_scan(uint32_t* a, uint32_t P, uint32_t* wave_tot) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nw = blockDim.x >> 6;
+    const uint32_t per = (P + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t local = 0;
+    for (uint32_t i = 0; i < per; ++i)
+        if (b0 + i < P) local += a[b0 + i];
+    uint32_t incl = local;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = uint32_t(__shfl_up(int(incl), d));
+        if (lane >= uint32_t(d)) incl += t;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t i = 0; i < wave; ++i) before += wave_tot[i];
+    uint32_t run = before + incl - local;
+    for (uint32_t i = 0; i < per; ++i)
+        if (b0 + i < P) {
+            const uint32_t x = a[b0 + i];
+            a[b0 + i] = run;
+            run += x;
+        }
+    if (threadIdx.x == blockDim.x - 1) {
+        uint32_t tot = 0;
+        for (uint32_t i = 0; i < nw; ++i) tot += wave_tot[i];
+        a[P] = tot;
+    }
+    __syncthreads();
+}
+
+// Roots with b(u) > kC4BlockBound: one workgroup each.
+//   0. b(u) <= kC4DirectBound: one pass counts every endpoint in the LDS
+//      table (at most half full);
+//   1. endpoints are bucketed by a hash into P ~ b(u) / kC4BucketKeys
+//      buckets of a fixed capacity in the workgroup's HBM scratch, as many
+//      buckets per round (one stream of the wedges) as the scratch holds,
+//      and each bucket is counted in the LDS table;
+//   2. if a bucket fills up, the remaining buckets take the exact path: a
+//      histogram, an exclusive scan of the bucket starts, and rounds of
+//      consecutive buckets whose endpoints fit the scratch.
+// A bucket whose distinct endpoints overfill the table (an insert finds
+// no slot within kC4MaxProbe of its home and flags it) is counted again in
+// 2, 4, ... parts split by further hash bits; a bucket larger than the
+// scratch is counted that way straight from the wedge stream.
+__global__ __launch_bounds__(kC4Block) void census_c4_heavy_kernel(const uint64_t* __restrict__ rowptr,
+                                                                   const uint32_t* __restrict__ col,
+                                                                   const uint32_t* __restrict__ tasks,
+                                                                   const uint64_t* __restrict__ bound,
+                                                                   uint32_t ntasks, uint32_t* __restrict__ cursor,
+                                                                   uint32_t* __restrict__ scratch_slab,
+                                                                   uint32_t scratch_keys, uint32_t max_slots,
+                                                                   uint32_t direct_bound,
+                                                                   unsigned long long* __restrict__ sums) {
+    __shared__ uint32_t sKey[kC4HeavySlots], sCnt[kC4HeavySlots];
+    __shared__ uint32_t sOff[kC4MaxBuckets + 1];
+    __shared__ uint32_t sWave[kC4Block / 64];
+    __shared__ uint32_t sList[kC4Block / 64][129];
+    __shared__ uint32_t sTask, sB1;
+    __shared__ uint32_t sOver[2];  // overflow flag per bucket part, alternating
+    uint32_t* scratch = scratch_slab + size_t(blockIdx.x) * scratch_keys;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nw = blockDim.x >> 6;
+    unsigned long long acc = 0;
+    // the bucket table starts clear and every part clears what it used
+    for (uint32_t i = threadIdx.x; i < kC4HeavySlots; i += blockDim.x) {
+        sKey[i] = kEmpty;
+        sCnt[i] = 0;
+    }
+    if (threadIdx.x == 0) sOver[0] = sOver[1] = 0;
+    uint32_t flip = 0;
+    while (true) {
+        if (threadIdx.x == 0) sTask = atomicAdd(cursor, 1u);
+        __syncthreads();
+        const uint32_t t = wave_bcast(sTask);
+        if (t >= ntasks) break;
+        const uint32_t u = wave_bcast(tasks[t]);
+        const uint64_t ub = ldg(rowptr, u);
+        const uint32_t du = uint32_t(ldg(rowptr, u + 1) - ub);
+        const uint32_t pre = lower_bound_global(col + ub, du, u);
+        const uint64_t bu = bound[u];
+        uint32_t P = 1, pbits = 0;
+        while (P < kC4MaxBuckets && uint64_t(P) * kC4BucketKeys < bu) {
+            P <<= 1;
+            ++pbits;
+        }
+        auto bucket_of = [&](uint32_t w) { return pbits ? (w * kC4BucketMul) >> (32 - pbits) : 0u; };
+        // part of w among `parts` (a power of two) inside its bucket
+        auto part_of = [&](uint32_t w, uint32_t parts) {
+            return parts > 1 ? ((w * kC4BucketMul) << pbits) >> (32 - __builtin_ctz(parts)) : 0u;
+        };
+        // Counts the endpoints of bucket b (keys[0..k) of the scratch, or the
+        // wedge stream when keys is null; every endpoint for kAllBuckets) into
+        // acc.  Returns false, counting nothing, when a split into more than
+        // max_parts parts would be needed.
+        auto count_bucket = [&](uint32_t b, const uint32_t* keys, uint32_t k, uint32_t max_parts) {
+            uint32_t parts = 1;
+            while (true) {
+                unsigned long long bacc = 0;
+                bool over = false;
+                for (uint32_t part = 0; part < parts && !over; ++part) {
+                    const uint32_t want = (k + parts - 1) / parts;
+                    uint32_t H, hshift;
+                    c4_geometry(min(want, max_slots / 2), &H, &hshift);
+                    uint32_t& ovf = sOver[flip];
+                    // A key is placed within kC4MaxProbe slots of its home
+                    // slot or the part is flagged (the table is too full):
+                    // lookups then stop there too, and a flagged part is
+                    // counted again, split further.  (No occupancy counter:
+                    // one LDS word taking every insert serialises them.)
+                    const uint32_t max_probe = min(H, kC4MaxProbe);
+                    auto add = [&](uint32_t w) {
                         uint32_t h = hash_slot(w, hshift);
-                        while (true) {
-                            const uint32_t key = sKey[h];
-                            if (key == w) break;
-                            if (key == kEmpty) {
+                        for (uint32_t probes = 0; probes < max_probe; ++probes) {
+                            const uint32_t kk = sKey[h];
+                            if (kk == w) {
+                                atomicAdd(&sCnt[h], 1u);
+                                return;
+                            }
+                            if (kk == kEmpty) {
                                 const uint32_t old = atomicCAS(&sKey[h], kEmpty, w);
-                                if (old == kEmpty || old == w) break;
+                                if (old == kEmpty || old == w) {
+                                    atomicAdd(&sCnt[h], 1u);
+                                    return;
+                                }
                             }
                             h = (h + 1) & (H - 1);
                         }
-                        atomicAdd(&sCnt[h], 1u);
+                        ovf = 1;
+                    };
+                    if (keys) {
+                        // four keys per thread loaded together
+                        for (uint32_t i0 = threadIdx.x; i0 < k; i0 += 4 * blockDim.x) {
+                            uint32_t kw[4];
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const uint32_t i = i0 + c * blockDim.x;
+                                kw[c] = i < k ? keys[i] : kEmpty;
+                            }
+#pragma unroll
+                            for (int c = 0; c < 4; ++c)
+                                if (kw[c] != kEmpty && part_of(kw[c], parts) == part) add(kw[c]);
+                        }
+                    } else {
+                        c4_wave_wedges(rowptr, col, ub, pre, u, wave * 64, nw * 64, sList[wave], [&](uint32_t w) {
+                            if ((b == kAllBuckets || bucket_of(w) == b) && part_of(w, parts) == part) add(w);
+                        });
                     }
-                    if (__ballot(w >= u)) break;  // sorted lists: the rest ranks above u
+                    __syncthreads();
+                    over = ovf != 0;
+                    for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) {
+                        const uint64_t c = sCnt[i];
+                        if (!over) bacc += c * (c - 1) / 2;
+                        sKey[i] = kEmpty;
+                        sCnt[i] = 0;
+                    }
+                    // the other flag pair was last read before the barrier
+                    // that opened this part
+                    if (threadIdx.x == 0) sOver[flip ^ 1] = 0;
+                    flip ^= 1;
+                    __syncthreads();
                 }
+                if (!over) {
+                    acc += bacc;
+                    return true;
+                }
+                if (parts >= max_parts) return false;
+                parts *= 2;  // this split left a part above the table: split further
+            }
+        };
+        // Roots with at most half as many wedges as the table has slots are
+        // counted in one pass over the whole table (split only if a probe
+        // run grows past the limit).
+        if (bu <= direct_bound && count_bucket(kAllBuckets, nullptr, uint32_t(bu), 2)) continue;
+        // When the scratch holds every bucket at a fixed capacity (1.5 times
+        // the mean the bound allows, plus 64), one stream appends each
+        // endpoint to its bucket; a bucket that fills up sends the root to
+        // the exact path below, which larger roots take directly.
+        const uint64_t cap64 = bu * 3 / (2 * uint64_t(P)) + 64;
+        bool exact = cap64 * P > scratch_keys;
+        if (!exact) {
+            const uint32_t cap = uint32_t(cap64);
+            for (uint32_t b = threadIdx.x; b < P; b += blockDim.x) sOff[b] = 0;
+            if (threadIdx.x == 0) sB1 = 0;  // a bucket filled up
+            __syncthreads();
+            c4_wave_wedges(rowptr, col, ub, pre, u, wave * 64, nw * 64, sList[wave], [&](uint32_t w) {
+                const uint32_t b = bucket_of(w);
+                const uint32_t pos = atomicAdd(&sOff[b], 1u);
+                if (pos < cap) scratch[b * cap + pos] = w;
+                else sB1 = 1;
+            });
+            __syncthreads();
+            exact = sB1 != 0;
+            if (!exact)
+                for (uint32_t b = 0; b < P; ++b) {
+                    const uint32_t k = sOff[b];
+                    if (k > 1) count_bucket(b, scratch + b * cap, k, ~0u);
+                }
+        }
+        if (!exact) continue;
+        // Exact path: histogram, exclusive scan of the bucket starts, then
+        // rounds of consecutive buckets whose endpoints fit the scratch.
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b <= P; b += blockDim.x) sOff[b] = 0;
+        __syncthreads();
+        c4_wave_wedges(rowptr, col, ub, pre, u, wave * 64, nw * 64, sList[wave],
+                       [&](uint32_t w) { atomicAdd(&sOff[bucket_of(w)], 1u); });
+        __syncthreads();
+        block_exclusive_scan(sOff, P, sWave);
+        uint32_t b0 = 0;
+        while (b0 < P) {
+            const uint32_t base0 = sOff[b0];
+            if (threadIdx.x == 0) {
+                // the last b1 with sOff[b1] - base0 <= scratch (starts are sorted)
+                uint32_t lo = b0, hi = P;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (sOff[mid] - base0 <= scratch_keys) lo = mid;
+                    else hi = mid - 1;
+                }
+                sB1 = lo;
             }
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) {
-                const uint64_t c = sCnt[i];
-                acc += c * (c - 1) / 2;
+            const uint32_t b1 = sB1;
+            if (b1 == b0) {
+                // one bucket above the scratch: count it from the wedge stream
+                const uint32_t k = sOff[b0 + 1] - base0;
+                count_bucket(b0, nullptr, k, ~0u);
+                b0 += 1;
+                continue;
             }
+            c4_wave_wedges(rowptr, col, ub, pre, u, wave * 64, nw * 64, sList[wave], [&](uint32_t w) {
+                const uint32_t b = bucket_of(w);
+                if (b >= b0 && b < b1) scratch[atomicAdd(&sOff[b], 1u) - base0] = w;
+            });
             __syncthreads();
-            continue;
-        }
-        for (uint32_t k = wave; k < pre; k += nw) {
-            const uint32_t v = ldg(col, ub + k);
-            const uint64_t vb = ldg(rowptr, v);
-            const uint32_t dv = uint32_t(ldg(rowptr, v + 1) - vb);
-            for (uint32_t base = 0; base < dv; base += 64) {
-                const uint32_t i = base + lane;
-                const uint32_t w = i < dv ? ldg(col, vb + i) : 0xFFFFFFFFu;
-                if (w < u && atomicAdd(&cnt[w], 1u) == 0) touched[atomicAdd(&sTouched, 1u)] = w;
-                if (__ballot(w >= u)) break;  // sorted lists: the rest ranks above u
+            // sOff[b] now holds the end of bucket b, i.e. the start of b + 1
+            for (uint32_t b = b0; b < b1; ++b) {
+                const uint32_t s0 = b == b0 ? base0 : sOff[b - 1];
+                const uint32_t k = sOff[b] - s0;
+                if (k > 1) count_bucket(b, scratch + (s0 - base0), k, ~0u);
             }
+            b0 = b1;
         }
-        __syncthreads();
-        const uint32_t nt = sTouched;
-        for (uint32_t s = threadIdx.x; s < nt; s += blockDim.x) {
-            const uint32_t w = touched[s];
-            const uint64_t c = cnt[w];
-            acc += c * (c - 1) / 2;
-            cnt[w] = 0;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) sTouched = 0;
     }
     const unsigned long long x = wave_sum_u64(acc);
     if (lane == 0 && x) atomicAdd(&sums[6], x);
@@ -460,7 +839,74 @@ static int build_ranked(DeviceGraph& g, Ranked& R) {
     return KZ_OK;
 }
 
-// Task lists, per-edge arrays, counter slabs and the 4-clique job.
+// 4-cycle pass: wedge bounds, the three root lists (largest bound first),
+// and the heavy kernel's endpoint scratch (one region per workgroup, one
+// workgroup per CU, fewer when the regions would take more than a quarter
+// of free memory).
+static int prepare_c4(Ranked& R) {
+    const Runtime& rt_ = rt();
+    hipStream_t s = rt_.stream;
+    const uint32_t n = R.n;
+    KZ_HIP(hipMalloc(&R.c4_bound, 8 * std::max<size_t>(1, n)));
+    if (n)
+        hipLaunchKernelGGL(c4_bound_kernel, dim3(std::max(1, rt_.num_cus * 8)), dim3(256), 0, s, R.rowptr, R.col,
+                           R.deg, n, R.c4_bound);
+    KZ_HIP(hipGetLastError());
+    std::vector<uint64_t> b(n);
+    if (n) KZ_HIP(hipMemcpyAsync(b.data(), R.c4_bound, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+    KZ_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> wave, block, heavy;
+    for (uint32_t u = 0; u < n; ++u) {
+        if (b[u] < 2) continue;  // fewer than two wedges close no 4-cycle
+        if (b[u] <= kC4WaveBound) wave.push_back(u);
+        else if (b[u] <= kC4BlockBound) block.push_back(u);
+        else heavy.push_back(u);
+    }
+    auto by_bound = [&](uint32_t x, uint32_t y) { return b[x] > b[y]; };
+    std::stable_sort(wave.begin(), wave.end(), by_bound);
+    std::stable_sort(block.begin(), block.end(), by_bound);
+    std::stable_sort(heavy.begin(), heavy.end(), by_bound);
+    R.n_c4_wave = uint32_t(wave.size());
+    R.n_c4_block = uint32_t(block.size());
+    R.n_c4_heavy = uint32_t(heavy.size());
+    std::vector<uint32_t> all(wave);
+    all.insert(all.end(), block.begin(), block.end());
+    all.insert(all.end(), heavy.begin(), heavy.end());
+    KZ_HIP(hipMalloc(&R.c4_tasks, 4 * std::max<size_t>(1, all.size())));
+    if (!all.empty()) KZ_HIP(hipMemcpy(R.c4_tasks, all.data(), 4 * all.size(), hipMemcpyHostToDevice));
+    R.bytes += 8 * size_t(n) + 4 * all.size();
+    // KZ_C4_SCRATCH_KEYS / KZ_C4_BUCKET_SLOTS / KZ_C4_DIRECT_BOUND shrink
+    // the scratch, the bucket table and the unbucketed attempt (tests use
+    // them to reach every path on small graphs)
+    R.c4_scratch_keys = kC4ScratchKeys;
+    R.c4_max_slots = kC4HeavySlots;
+    R.c4_direct_bound = kC4DirectBound;
+    if (const char* e = std::getenv("KZ_C4_DIRECT_BOUND"))
+        R.c4_direct_bound = uint32_t(std::min<unsigned long>(kC4DirectBound, std::strtoul(e, nullptr, 10)));
+    if (const char* e = std::getenv("KZ_C4_SCRATCH_KEYS"))
+        R.c4_scratch_keys = uint32_t(std::min<unsigned long>(kC4ScratchKeys, std::max<unsigned long>(1, std::strtoul(e, nullptr, 10))));
+    if (const char* e = std::getenv("KZ_C4_BUCKET_SLOTS")) {
+        uint32_t h = 64;
+        const unsigned long want = std::strtoul(e, nullptr, 10);
+        while (h < kC4HeavySlots && h < want) h <<= 1;
+        R.c4_max_slots = h;
+    }
+    if (R.n_c4_heavy) {
+        size_t free_b = 0, total_b = 0;
+        KZ_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t per_wg = size_t(R.c4_scratch_keys) * 4;
+        R.c4_heavy_grid = uint32_t(std::max<size_t>(
+            1, std::min<size_t>(std::min<size_t>(size_t(rt_.num_cus), R.n_c4_heavy), free_b / 4 / per_wg)));
+        if (hipMalloc(&R.c4_scratch, per_wg * R.c4_heavy_grid) != hipSuccess) {
+            (void)hipGetLastError();
+            KZ_FAIL(KZ_ERR_NOMEM, "4-cycle endpoint scratch (%zu bytes)", per_wg * R.c4_heavy_grid);
+        }
+        R.bytes += per_wg * R.c4_heavy_grid;
+    }
+    return KZ_OK;
+}
+
+// Task lists, per-edge arrays, the 4-cycle pass and the 4-clique job.
 static int prepare_passes(Ranked& R) {
     const Runtime& rt_ = rt();
     std::vector<uint32_t> small, large;
@@ -489,24 +935,10 @@ static int prepare_passes(Ranked& R) {
     if (!all.empty()) KZ_HIP(hipMemcpy(R.tasks, all.data(), 4 * all.size(), hipMemcpyHostToDevice));
     KZ_HIP(hipMalloc(&R.troot, 4 * std::max<uint64_t>(1, R.fnnz)));
     KZ_HIP(hipMalloc(&R.tmid, 4 * std::max<uint64_t>(1, R.fnnz)));
-    KZ_HIP(hipMalloc(&R.cursor, 4 * 4 * kCursorStride));
+    KZ_HIP(hipMalloc(&R.cursor, 4 * kCursors * kCursorStride));
     KZ_HIP(hipMalloc(&R.sums, 8 * 8));
-    // 4-cycle workgroups: two per CU, each with 8 bytes per vertex of slab,
-    // fewer when the slabs would take more than a quarter of free memory
-    size_t free_b = 0, total_b = 0;
-    KZ_HIP(hipMemGetInfo(&free_b, &total_b));
-    const size_t per_wg = size_t(std::max<uint32_t>(1, R.n)) * 8;
-    R.c4_grid = uint32_t(std::max<size_t>(1, std::min<size_t>(size_t(rt_.num_cus) * 2, free_b / 4 / per_wg)));
-    const size_t slab = size_t(R.c4_grid) * std::max<uint32_t>(1, R.n) * 4;
-    R.bytes += 2 * slab + 8 * (2 * std::max<uint64_t>(1, R.fnnz) + all.size() + 8);
-    if (hipMalloc(&R.cnt, slab) != hipSuccess || hipMalloc(&R.touched, slab) != hipSuccess) {
-        (void)hipGetLastError();
-        KZ_FAIL(KZ_ERR_NOMEM, "4-cycle counter slabs (%zu bytes each)", slab);
-    }
-    // counters start at zero and every pass leaves them at zero
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1024), dim3(256), 0, rt_.stream, R.cnt, uint64_t(slab / 4));
-    KZ_HIP(hipGetLastError());
-    KZ_HIP(hipStreamSynchronize(rt_.stream));
+    R.bytes += 8 * (2 * std::max<uint64_t>(1, R.fnnz) + all.size() + 8);
+    if (int e = prepare_c4(R)) return e;
     std::vector<CliqueQuery> q = {CliqueQuery{R.frowptr, R.fcol, &R.h_fdeg, 4}};
     if (int e = clique_job_prepare(R.k4, q, default_buckets())) return e;
     R.bytes += R.k4.bytes;
@@ -534,7 +966,7 @@ static int run_census(Ranked& R, uint64_t* sums, double* device_ms) {
     KZ_HIP(hipEventCreate(&e0));
     KZ_HIP(hipEventCreate(&e1));
     KZ_HIP(hipEventRecord(e0, s));
-    hipLaunchKernelGGL(fill_small_kernel, dim3(1), dim3(64), 0, s, R.cursor, int(4 * kCursorStride), R.sums, 8);
+    hipLaunchKernelGGL(fill_small_kernel, dim3(1), dim3(64), 0, s, R.cursor, int(kCursors * kCursorStride), R.sums, 8);
     const uint64_t fz = std::max<uint64_t>(1, R.fnnz);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(2048), dim3(256), 0, s, R.troot, fz);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(2048), dim3(256), 0, s, R.tmid, fz);
@@ -566,12 +998,19 @@ static int run_census(Ranked& R, uint64_t* sums, double* device_ms) {
     }
     hipLaunchKernelGGL(census_sums_kernel, dim3(std::max(1, rt_.num_cus * 4)), dim3(256), 0, s, R.frowptr, R.fcol,
                        R.deg, R.n, R.troot, R.tmid, R.sums);
-    if (R.n)
-        // 16 waves per workgroup: the counter slabs bound the workgroups
-        // (two per CU), and the HBM counters' returning atomics need many
-        // waves in flight
-        hipLaunchKernelGGL(census_c4_kernel<kC4Block>, dim3(R.c4_grid), dim3(kC4Block), 0, s, R.rowptr, R.col, R.n,
-                           R.cursor + 2 * kCursorStride, R.cnt, R.touched, R.sums);
+    if (R.n_c4_heavy)
+        hipLaunchKernelGGL(census_c4_heavy_kernel, dim3(R.c4_heavy_grid), dim3(kC4Block), 0, s, R.rowptr, R.col,
+                           R.c4_tasks + R.n_c4_wave + R.n_c4_block, R.c4_bound, R.n_c4_heavy,
+                           R.cursor + 4 * kCursorStride, R.c4_scratch, R.c4_scratch_keys, R.c4_max_slots,
+                           R.c4_direct_bound, R.sums);
+    if (R.n_c4_block)
+        hipLaunchKernelGGL(census_c4_block_kernel, dim3(std::min<uint32_t>(R.n_c4_block, uint32_t(rt_.num_cus) * 2)),
+                           dim3(kC4Block), 0, s, R.rowptr, R.col, R.c4_tasks + R.n_c4_wave, R.c4_bound, R.n_c4_block,
+                           R.cursor + 3 * kCursorStride, R.sums);
+    if (R.n_c4_wave)
+        hipLaunchKernelGGL(census_c4_wave_kernel,
+                           dim3(std::min<uint32_t>((R.n_c4_wave + 3) / 4, uint32_t(rt_.num_cus) * 4)), dim3(256), 0, s,
+                           R.rowptr, R.col, R.c4_tasks, R.c4_bound, R.n_c4_wave, R.cursor + 2 * kCursorStride, R.sums);
     KZ_HIP(hipGetLastError());
     KZ_HIP(hipEventRecord(e1, s));
     unsigned long long h[8] = {0};
@@ -629,8 +1068,8 @@ int motif_census(DeviceGraph& d, uint64_t* counts, kz_run_stats* stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->device_ms = ms;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        stats->launches = 6;
-        stats->tasks = uint64_t(R->nsmall) + R->nlarge + R->n;
+        stats->launches = 8;
+        stats->tasks = uint64_t(R->nsmall) + R->nlarge + R->n_c4_wave + R->n_c4_block + R->n_c4_heavy;
         stats->embeddings[2] = uint64_t(d.nnz) / 2;
         stats->embeddings[3] = T;
         stats->embeddings[4] = iK4;

@@ -52,6 +52,32 @@ def test_census_matches_oracle(kzdev, name):
     dg.release()
 
 
+@pytest.mark.parametrize("scratch,slots,direct", [("3000", None, "0"), ("500", None, "0"), (None, "64", None),
+                                                  ("700", "128", "0"), (None, None, "0"), ("20000", "256", "2000")])
+def test_census_heavy_root_paths(kzdev, monkeypatch, scratch, slots, direct):
+    """Roots above the workgroup hash first try one pass over the LDS table
+    (direct); the others bucket their wedge endpoints through an HBM scratch
+    region.  A small region makes them take several rounds and fill buckets
+    (the exact path), and buckets larger than the region are counted from
+    the wedge stream.  A small table makes the counts split buckets into
+    parts and makes direct attempts fail."""
+    kz = kzdev
+    knobs = {"KZ_C4_SCRATCH_KEYS": scratch, "KZ_C4_BUCKET_SLOTS": slots, "KZ_C4_DIRECT_BOUND": direct}
+    for name in ("dense_er", "chunglu_20k", "rmat13"):
+        g = dict(_medium(kz))[name]
+        want = oracle.motif_census(g.rowptr, g.col)
+        for k, v in knobs.items():
+            if v is not None:
+                monkeypatch.setenv(k, v)
+        print(f"[case] census {name} {knobs}", flush=True)
+        dg = g.to_device()  # the settings are read when the census state is built
+        got = dg.motif_census()
+        dg.release()
+        for k in knobs:
+            monkeypatch.delenv(k, raising=False)
+        assert got == want, (name, knobs)
+
+
 def test_census_rejects_oriented_graph(kzdev):
     kz = kzdev
     dg = kz.gen_erdos_renyi(500, 3000, 1).oriented().to_device()
