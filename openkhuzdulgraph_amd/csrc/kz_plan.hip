@@ -170,14 +170,24 @@ __device__ uint64_t count_classes(const uint32_t* A, uint32_t n, const uint32_t*
     return c;
 }
 
-__device__ __forceinline__ bool hash_has(const uint32_t* hk, uint32_t hmask, uint32_t hshift, uint32_t x) {
-    uint32_t h = (x * 2654435761u) >> hshift;
+// Whether x is in the LDS hash, probing from slot h on.
+__device__ __forceinline__ bool hash_find_from(const uint32_t* hk, uint32_t hmask, uint32_t h, uint32_t x) {
     while (true) {
         const uint32_t key = hk[h];
         if (key == x) return true;
         if (key == kEmpty) return false;
         h = (h + 1) & hmask;
     }
+}
+
+// Whether x is in the LDS hash: the first two probes branch-free (the
+// table is sparse, so the loop rarely runs).
+__device__ __forceinline__ bool hash_has2(const uint32_t* hk, uint32_t hmask, uint32_t hshift, uint32_t x) {
+    const uint32_t h = (x * 2654435761u) >> hshift;
+    const uint32_t k0 = hk[h], k1 = hk[(h + 1) & hmask];
+    const bool found = k0 == x || (k0 != kEmpty && k1 == x);
+    if (found || k0 == kEmpty || k1 == kEmpty) return found;
+    return hash_find_from(hk, hmask, (h + 2) & hmask, x);
 }
 
 // Rebuild embedding e of size S: matched vertices m[0..S) and ancestor
@@ -428,13 +438,14 @@ __device__ void flat_segment(const LevelArgs& a, const kz_step_list& L, uint32_t
         fl.off[lane] = incl - len;
         fl.start[lane] = start;
         fl.w[lane] = wdup ? kEmpty : w;
-        // P's hash: at least 4 slots per key where the table allows (short
-        // probe runs); only this path fills the table in the flat kernels
+        // P's hash: 16 slots per key where the table allows (probe runs of
+        // one or two slots); only this path fills the table in the flat
+        // kernels
         uint32_t hmask = 0, hshift = 0;
         const bool probe = has_ops && P.n > 0;
         const bool hashed = probe && P.n <= kHashCap / 2;
         if (hashed) {
-            hash_geometry(P.n <= kHashCap / 4 ? 2 * P.n : P.n, &hmask, &hshift);
+            hash_geometry(min(8 * P.n, kHashCap / 2), &hmask, &hshift);
             if (!(hc.p == P.p && hc.n == P.n)) {
                 for (uint32_t i = lane; i <= hmask; i += 64) hk[i] = kEmpty;
                 wave_lds_sync();
@@ -459,17 +470,63 @@ __device__ void flat_segment(const LevelArgs& a, const kz_step_list& L, uint32_t
             cls_out[o] = (md == 2 || md == 3) ? pred : 0u;
         }
         uint32_t c[KZ_MAX_OUTS] = {0, 0, 0, 0};  // this lane's counts
-        // kFlatUnroll chunks of 64 positions per pass, all their list loads in
-        // flight before the first probe.  Branch-free: positions past the end
-        // are clamped to the last one and not counted.
+        if (hashed || !probe) {
+        // Each lane walks its own contiguous range of ceil(T / 64) positions
+        // of the concatenation (its first sibling found once by binary
+        // search), kFlatUnroll loads in flight, moving to the next sibling
+        // when it reaches its offset: no per-position search.
+        const uint32_t q = (T + 63) / 64;
+        uint32_t p = min(lane * q, T);
+        const uint32_t e = min(p + q, T);
+        uint32_t lo = b;  // the last sibling whose offset is <= p holds p (non-empty)
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1) {
+            const uint32_t t = lo + step;
+            if (t < en && fl.off[t] <= p) lo = t;
+        }
+        uint32_t nxt = lo + 1 < en ? fl.off[lo + 1] : T;
+        uint64_t adr = fl.start[lo] - fl.off[lo];  // col index of position p is adr + p (mod 2^64)
+        uint32_t wv = fl.w[lo];
+        // Each step takes up to kFlatUnroll positions of one sibling (the
+        // loads clamped, not branched, to the sibling's end), so a lane
+        // checks for the next sibling once per step.
+        while (p < e) {
+            if (p == nxt) {
+                do {
+                    ++lo;
+                    nxt = lo + 1 < en ? fl.off[lo + 1] : T;
+                } while (nxt == p);
+                adr = fl.start[lo] - fl.off[lo];
+                wv = fl.w[lo];
+            }
+            const uint32_t m = min(uint32_t(kFlatUnroll), min(e, nxt) - p);
+            uint32_t xs[kFlatUnroll];
+#pragma unroll
+            for (int k = 0; k < kFlatUnroll; ++k) xs[k] = a.col[adr + p + (uint32_t(k) < m ? uint32_t(k) : 0u)];
+#pragma unroll
+            for (int k = 0; k < kFlatUnroll; ++k) {
+                bool found = false;
+                if (probe) found = hash_has2(hk, hmask, hshift, xs[k]);
+                const uint32_t cls = pos_class_plus<S - 1>(xs[k], zs, wv);
+                const uint32_t fm = 0u - uint32_t(found);  // (a select of arrays would index them)
+                const uint32_t in = uint32_t(k) < m;
+#pragma unroll
+                for (int o = 0; o < KZ_MAX_OUTS; ++o)
+                    c[o] += in & ((((cls_in[o] & fm) | (cls_out[o] & ~fm)) >> cls) & 1u);
+            }
+            p += m;
+        }
+        } else {
+        // P probed by binary search in HBM: 64 consecutive positions per
+        // chunk, so the lanes' searches (values of one sorted list) follow
+        // nearly the same path; kFlatUnroll chunks loaded together, each
+        // position's sibling found by binary lifting over the offsets.
         uint32_t lo = b;  // sibling of this lane's previous position (positions only grow)
         for (uint32_t base = 0; base < T; base += 64 * kFlatUnroll) {
             uint32_t xs[kFlatUnroll], ws[kFlatUnroll];
 #pragma unroll
             for (int k = 0; k < kFlatUnroll; ++k) {
                 const uint32_t p = min(base + 64 * k + lane, T - 1);
-                // the last sibling whose offset is <= p holds p (its list is
-                // non-empty); siblings lie in [lo, en), en - lo <= 64
 #pragma unroll
                 for (uint32_t step = 32; step; step >>= 1) {
                     const uint32_t t = lo + step;
@@ -482,13 +539,14 @@ __device__ void flat_segment(const LevelArgs& a, const kz_step_list& L, uint32_t
 #pragma unroll
             for (int k = 0; k < kFlatUnroll; ++k) {
                 const bool valid = base + 64 * k + lane < T;
-                bool found = false;
-                if (probe) found = hashed ? hash_has(hk, hmask, hshift, xs[k]) : bsearch_global(P.p, P.n, xs[k]);
+                const bool found = bsearch_global(P.p, P.n, xs[k]);
                 const uint32_t cls = pos_class_plus<S - 1>(xs[k], zs, ws[k]);
+                const uint32_t fm = 0u - uint32_t(found);
 #pragma unroll
                 for (int o = 0; o < KZ_MAX_OUTS; ++o)
-                    c[o] += valid && (((found ? cls_in[o] : cls_out[o]) >> cls) & 1u);
+                    c[o] += valid && ((((cls_in[o] & fm) | (cls_out[o] & ~fm)) >> cls) & 1u);
             }
+        }
         }
 #pragma unroll
         for (int o = 0; o < KZ_MAX_OUTS; ++o) {
