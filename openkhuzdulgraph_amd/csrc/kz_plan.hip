@@ -30,6 +30,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "kz_device.h"
 #include "kz_kernels.h"
@@ -86,13 +87,16 @@ struct LevelArgs {
 };
 
 // Per-wave LDS of the flat path: each sibling lane's stream offset, list
-// start and new vertex (kEmpty when it repeats an ancestor's vertex).
+// start and new vertex (kEmpty when it repeats an ancestor's vertex); with
+// SIZES, per sibling its children and its cached results' sizes.
+template <bool SIZES>
 struct FlatLds {
     uint64_t start[64];
     uint32_t off[64];
     uint32_t w[64];
     uint64_t anc_idx[KZ_MAX_PATTERN];  // the segment's ancestors: indexes at sizes 1.. and vertices
     uint32_t anc_m[KZ_MAX_PATTERN];
+    uint32_t sib[SIZES ? KZ_MAX_OPS + 1 : 1][64];
 };
 
 __device__ __forceinline__ void add_agg(unsigned long long* acc, int agg, unsigned long long v) {
@@ -348,13 +352,17 @@ __device__ void one_embedding(const LevelArgs& a, uint64_t e, uint32_t* hk, Hash
 // offsets), probing P's LDS hash once per element; counts of ancestor sets
 // (and the P part of P \ N(w)) are class runs found by binary search.  The
 // counts equal the per-embedding path's: the outputs only add up.
-template <int S>
+template <int S, bool SIZES>
 __device__ void flat_segment(const LevelArgs& a, const kz_step_list& L, uint32_t b, uint32_t en, const uint32_t* m,
-                             const uint64_t* idx, uint64_t wstart, uint32_t wlen, uint32_t* hk, HashCache& hc,
-                             FlatLds& fl, unsigned long long* acc, unsigned long long* lacc) {
+                             const uint64_t* idx, uint64_t wstart, uint32_t wlen, uint64_t e_lane, uint32_t* hk,
+                             HashCache& hc, FlatLds<SIZES>& fl, unsigned long long* acc, unsigned long long* lacc) {
     static_assert(S >= 2, "the flat path needs an ancestor");
     const uint32_t lane = threadIdx.x & 63;
     const bool sib = lane >= b && lane < en;
+    if constexpr (SIZES) {
+#pragma unroll
+        for (int r = 0; r < KZ_MAX_OPS + 1; ++r) fl.sib[r][lane] = 0;
+    }
     // ancestors (the same on every lane of the segment), published by its
     // first lane for operand lookups by plan index
     wave_lds_sync();
@@ -406,23 +414,45 @@ __device__ void flat_segment(const LevelArgs& a, const kz_step_list& L, uint32_t
         const kz_op& o0 = L.ops[0];
         P = resolve_shared(varying(o0.lhs) ? o0.rhs : o0.lhs);
     }
-    uint32_t omode[KZ_MAX_OUTS] = {0, 0, 0, 0};
+    // Counters: one per output, and with SIZES one per cached op result
+    // (its size).  Each goes to an aggregator (COUNT outputs: dest -1), to
+    // the sibling's children (SCATTER outputs: dest 0) or to the sibling's
+    // size of cached result j (dest 1 + j); -2: unused.
+    constexpr int NC = SIZES ? KZ_MAX_OUTS + KZ_MAX_OPS : KZ_MAX_OUTS;
+    auto op_mode = [&](const kz_op& op) -> uint32_t {
+        if (op.op == KZ_OP_INTERSECT) return 1;
+        return varying(op.lhs) ? 2u : (1u | 4u);
+    };
+    uint32_t omode[NC], opred[NC];
+    int cdest[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        omode[k] = 0;
+        opred[k] = 0;
+        cdest[k] = -2;
+    }
     bool need_stream = false;
 #pragma unroll
     for (int o = 0; o < KZ_MAX_OUTS; ++o) {
         if (o >= L.num_outs) break;
         const kz_out& out = L.outs[o];
         uint32_t md = 0;
-        if (out.src.kind == KZ_SET_SLOT && out.src.a == S) {
-            const kz_op& op = L.ops[out.src.b];
-            if (op.op == KZ_OP_INTERSECT) md = 1;
-            else if (varying(op.lhs)) md = 2;
-            else md = 1 | 4;
-        } else if (varying(out.src)) {
-            md = 3;
-        }
+        if (out.src.kind == KZ_SET_SLOT && out.src.a == S) md = op_mode(L.ops[out.src.b]);
+        else if (varying(out.src)) md = 3;
         omode[o] = md;
+        opred[o] = out.pred;
+        cdest[o] = out.kind == KZ_OUT_COUNT ? -1 : 0;
         need_stream |= (md & 3) != 0;
+    }
+    if constexpr (SIZES) {
+#pragma unroll
+        for (int j = 0; j < KZ_MAX_OPS; ++j) {
+            if (j >= L.num_ops || !(a.cached_mask & (1u << j))) continue;
+            omode[KZ_MAX_OUTS + j] = op_mode(L.ops[j]);
+            opred[KZ_MAX_OUTS + j] = ~0u;
+            cdest[KZ_MAX_OUTS + j] = 1 + j;
+            need_stream = true;
+        }
     }
     if (need_stream) {
         const uint64_t start = sib ? wstart : 0;
@@ -459,118 +489,183 @@ __device__ void flat_segment(const LevelArgs& a, const kz_step_list& L, uint32_t
             }
         }
         wave_lds_sync();
-        // per output, the position classes counted for a member of P and for
-        // a non-member (0 for outputs not streamed)
-        uint32_t cls_in[KZ_MAX_OUTS], cls_out[KZ_MAX_OUTS];
+        // The streamed counters, compacted: the loop below is compiled for
+        // 1, 2, 4 or NC of them (each costs a few VALU per position).
+        int sidx[NC];
+        int ns = 0;
 #pragma unroll
-        for (int o = 0; o < KZ_MAX_OUTS; ++o) {
-            const uint32_t md = omode[o] & 3u;
-            const uint32_t pred = o < L.num_outs ? L.outs[o].pred : 0u;
-            cls_in[o] = (md == 1 || md == 3) ? pred : 0u;
-            cls_out[o] = (md == 2 || md == 3) ? pred : 0u;
-        }
-        uint32_t c[KZ_MAX_OUTS] = {0, 0, 0, 0};  // this lane's counts
-        if (hashed || !probe) {
-        // Each lane walks its own contiguous range of ceil(T / 64) positions
-        // of the concatenation (its first sibling found once by binary
-        // search), kFlatUnroll loads in flight, moving to the next sibling
-        // when it reaches its offset: no per-position search.
-        const uint32_t q = (T + 63) / 64;
-        uint32_t p = min(lane * q, T);
-        const uint32_t e = min(p + q, T);
-        uint32_t lo = b;  // the last sibling whose offset is <= p holds p (non-empty)
+        for (int k = 0; k < NC; ++k) {
+            sidx[k] = -1;
+            if (omode[k] & 3u) {
 #pragma unroll
-        for (uint32_t step = 32; step; step >>= 1) {
-            const uint32_t t = lo + step;
-            if (t < en && fl.off[t] <= p) lo = t;
-        }
-        uint32_t nxt = lo + 1 < en ? fl.off[lo + 1] : T;
-        uint64_t adr = fl.start[lo] - fl.off[lo];  // col index of position p is adr + p (mod 2^64)
-        uint32_t wv = fl.w[lo];
-        // Each step takes up to kFlatUnroll positions of one sibling (the
-        // loads clamped, not branched, to the sibling's end), so a lane
-        // checks for the next sibling once per step.
-        while (p < e) {
-            if (p == nxt) {
-                do {
-                    ++lo;
-                    nxt = lo + 1 < en ? fl.off[lo + 1] : T;
-                } while (nxt == p);
-                adr = fl.start[lo] - fl.off[lo];
-                wv = fl.w[lo];
+                for (int t = 0; t < NC; ++t)
+                    if (t == ns) sidx[t] = k;
+                ++ns;
             }
-            const uint32_t m = min(uint32_t(kFlatUnroll), min(e, nxt) - p);
-            uint32_t xs[kFlatUnroll];
-#pragma unroll
-            for (int k = 0; k < kFlatUnroll; ++k) xs[k] = a.col[adr + p + (uint32_t(k) < m ? uint32_t(k) : 0u)];
-#pragma unroll
-            for (int k = 0; k < kFlatUnroll; ++k) {
-                bool found = false;
-                if (probe) found = hash_has2(hk, hmask, hshift, xs[k]);
-                const uint32_t cls = pos_class_plus<S - 1>(xs[k], zs, wv);
-                const uint32_t fm = 0u - uint32_t(found);  // (a select of arrays would index them)
-                const uint32_t in = uint32_t(k) < m;
-#pragma unroll
-                for (int o = 0; o < KZ_MAX_OUTS; ++o)
-                    c[o] += in & ((((cls_in[o] & fm) | (cls_out[o] & ~fm)) >> cls) & 1u);
-            }
-            p += m;
         }
-        } else {
-        // P probed by binary search in HBM: 64 consecutive positions per
-        // chunk, so the lanes' searches (values of one sorted list) follow
-        // nearly the same path; kFlatUnroll chunks loaded together, each
-        // position's sibling found by binary lifting over the offsets.
-        uint32_t lo = b;  // sibling of this lane's previous position (positions only grow)
-        for (uint32_t base = 0; base < T; base += 64 * kFlatUnroll) {
-            uint32_t xs[kFlatUnroll], ws[kFlatUnroll];
+        auto run_stream = [&](auto ns_tag) {
+            constexpr int NS = decltype(ns_tag)::value;
+            // per counter, the position classes counted for a member of P and
+            // for a non-member, where it goes, and whether it subtracts
+            uint32_t cls_in[NS], cls_out[NS], c[NS];  // c: this lane's counts
+            int dst[NS];
+            uint32_t neg[NS], agg[NS];
 #pragma unroll
-            for (int k = 0; k < kFlatUnroll; ++k) {
-                const uint32_t p = min(base + 64 * k + lane, T - 1);
+            for (int i = 0; i < NS; ++i) {
+                cls_in[i] = cls_out[i] = c[i] = neg[i] = agg[i] = 0;
+                dst[i] = -2;
+#pragma unroll
+                for (int k = 0; k < NC; ++k)
+                    if (k == sidx[i]) {
+                        const uint32_t md = omode[k] & 3u;
+                        cls_in[i] = (md == 1 || md == 3) ? opred[k] : 0u;
+                        cls_out[i] = (md == 2 || md == 3) ? opred[k] : 0u;
+                        dst[i] = cdest[k];
+                        neg[i] = omode[k] & 4u;
+                        agg[i] = k < KZ_MAX_OUTS && k < L.num_outs ? L.outs[k].aggregator : 0u;
+                    }
+            }
+            // SIZES: a lane's counts for its current sibling go to that
+            // sibling's LDS counters when it moves on
+            auto flush = [&](uint32_t sibling) {
+                if constexpr (SIZES) {
+#pragma unroll
+                    for (int i = 0; i < NS; ++i)
+                        if (dst[i] >= 0 && c[i]) {
+                            atomicAdd(&fl.sib[dst[i]][sibling], neg[i] ? 0u - c[i] : c[i]);
+                            c[i] = 0;
+                        }
+                }
+            };
+            if (hashed || !probe || SIZES) {
+                // Each lane walks its own contiguous range of ceil(T / 64)
+                // positions of the concatenation (its first sibling found once
+                // by binary search), kFlatUnroll loads in flight, moving to
+                // the next sibling when it reaches its offset: no per-position
+                // search.
+                const uint32_t q = (T + 63) / 64;
+                uint32_t p = min(lane * q, T);
+                const uint32_t e = min(p + q, T);
+                uint32_t lo = b;  // the last sibling whose offset is <= p holds p (non-empty)
 #pragma unroll
                 for (uint32_t step = 32; step; step >>= 1) {
                     const uint32_t t = lo + step;
-                    const uint32_t ot = fl.off[min(t, en - 1)];
-                    lo = (t < en && ot <= p) ? t : lo;
+                    if (t < en && fl.off[t] <= p) lo = t;
                 }
-                xs[k] = a.col[fl.start[lo] + (p - fl.off[lo])];
-                ws[k] = fl.w[lo];
+                uint32_t nxt = lo + 1 < en ? fl.off[lo + 1] : T;
+                uint64_t adr = fl.start[lo] - fl.off[lo];  // col index of position p is adr + p (mod 2^64)
+                uint32_t wv = fl.w[lo];
+                // Each step takes up to kFlatUnroll positions of one sibling
+                // (the loads clamped, not branched, to the sibling's end), so
+                // a lane checks for the next sibling once per step.
+                while (p < e) {
+                    if (p == nxt) {
+                        flush(lo);
+                        do {
+                            ++lo;
+                            nxt = lo + 1 < en ? fl.off[lo + 1] : T;
+                        } while (nxt == p);
+                        adr = fl.start[lo] - fl.off[lo];
+                        wv = fl.w[lo];
+                    }
+                    const uint32_t m = min(uint32_t(kFlatUnroll), min(e, nxt) - p);
+                    uint32_t xs[kFlatUnroll];
+#pragma unroll
+                    for (int k = 0; k < kFlatUnroll; ++k) xs[k] = a.col[adr + p + (uint32_t(k) < m ? uint32_t(k) : 0u)];
+#pragma unroll
+                    for (int k = 0; k < kFlatUnroll; ++k) {
+                        bool found = false;
+                        if (probe)
+                            found = hashed ? hash_has2(hk, hmask, hshift, xs[k]) : bsearch_global(P.p, P.n, xs[k]);
+                        const uint32_t cls = pos_class_plus<S - 1>(xs[k], zs, wv);
+                        const uint32_t fm = 0u - uint32_t(found);  // (a select of arrays would index them)
+                        const uint32_t in = uint32_t(k) < m;
+#pragma unroll
+                        for (int i = 0; i < NS; ++i)
+                            c[i] += in & ((((cls_in[i] & fm) | (cls_out[i] & ~fm)) >> cls) & 1u);
+                    }
+                    p += m;
+                }
+                flush(lo);
+            } else {
+                // P probed by binary search in HBM: 64 consecutive positions
+                // per chunk, so the lanes' searches (values of one sorted
+                // list) follow nearly the same path; kFlatUnroll chunks loaded
+                // together, each position's sibling found by binary lifting
+                // over the offsets.
+                uint32_t lo = b;  // sibling of this lane's previous position (positions only grow)
+                for (uint32_t base = 0; base < T; base += 64 * kFlatUnroll) {
+                    uint32_t xs[kFlatUnroll], ws[kFlatUnroll];
+#pragma unroll
+                    for (int k = 0; k < kFlatUnroll; ++k) {
+                        const uint32_t p = min(base + 64 * k + lane, T - 1);
+#pragma unroll
+                        for (uint32_t step = 32; step; step >>= 1) {
+                            const uint32_t t = lo + step;
+                            const uint32_t ot = fl.off[min(t, en - 1)];
+                            lo = (t < en && ot <= p) ? t : lo;
+                        }
+                        xs[k] = a.col[fl.start[lo] + (p - fl.off[lo])];
+                        ws[k] = fl.w[lo];
+                    }
+#pragma unroll
+                    for (int k = 0; k < kFlatUnroll; ++k) {
+                        const bool valid = base + 64 * k + lane < T;
+                        const bool found = bsearch_global(P.p, P.n, xs[k]);
+                        const uint32_t cls = pos_class_plus<S - 1>(xs[k], zs, ws[k]);
+                        const uint32_t fm = 0u - uint32_t(found);
+#pragma unroll
+                        for (int i = 0; i < NS; ++i)
+                            c[i] += valid && ((((cls_in[i] & fm) | (cls_out[i] & ~fm)) >> cls) & 1u);
+                    }
+                }
             }
 #pragma unroll
-            for (int k = 0; k < kFlatUnroll; ++k) {
-                const bool valid = base + 64 * k + lane < T;
-                const bool found = bsearch_global(P.p, P.n, xs[k]);
-                const uint32_t cls = pos_class_plus<S - 1>(xs[k], zs, ws[k]);
-                const uint32_t fm = 0u - uint32_t(found);
-#pragma unroll
-                for (int o = 0; o < KZ_MAX_OUTS; ++o)
-                    c[o] += valid && ((((cls_in[o] & fm) | (cls_out[o] & ~fm)) >> cls) & 1u);
-            }
-        }
-        }
-#pragma unroll
-        for (int o = 0; o < KZ_MAX_OUTS; ++o) {
-            if (!(omode[o] & 3u)) continue;
-            add_agg(lacc, L.outs[o].aggregator, (omode[o] & 4u) ? 0ull - c[o] : (unsigned long long)c[o]);
-        }
+            for (int i = 0; i < NS; ++i)
+                if (dst[i] == -1) add_agg(lacc, int(agg[i]), neg[i] ? 0ull - c[i] : (unsigned long long)c[i]);
+        };
+        if (ns <= 1) run_stream(std::integral_constant<int, 1>());
+        else if (ns <= 2) run_stream(std::integral_constant<int, 2>());
+        else if (ns <= 4) run_stream(std::integral_constant<int, 4>());
+        else run_stream(std::integral_constant<int, NC>());
     }
+    // per-sibling parts: counts of an ancestor set, and the P part of P \ N(w)
+    uint32_t extra[KZ_MAX_OPS + 1] = {0, 0, 0, 0, 0};  // SIZES: children, cached sizes
 #pragma unroll
     for (int o = 0; o < KZ_MAX_OUTS; ++o) {
         if (o >= L.num_outs) break;
         if ((omode[o] & 3u) && !(omode[o] & 4u)) continue;
         const kz_out& out = L.outs[o];
         const SetView A = (omode[o] & 4u) ? P : resolve_shared(out.src);
-        if (sib) add_agg(lacc, out.aggregator, count_classes<S>(A.p, A.n, zw, out.pred));
+        if (!sib) continue;
+        const uint64_t cnt = count_classes<S>(A.p, A.n, zw, out.pred);
+        if (cdest[o] == -1) add_agg(lacc, out.aggregator, cnt);
+        else extra[0] += uint32_t(cnt);
+    }
+    if constexpr (SIZES) {
+#pragma unroll
+        for (int j = 0; j < KZ_MAX_OPS; ++j)
+            if (omode[KZ_MAX_OUTS + j] & 4u) extra[1 + j] += P.n;
+        wave_lds_sync();
+        if (sib) {
+            if (a.nchild) a.nchild[e_lane - a.begin] = fl.sib[0][lane] + extra[0];
+#pragma unroll
+            for (int j = 0; j < KZ_MAX_OPS; ++j)
+                if (a.cached_mask & (1u << j))
+                    a.slot_len_out[j][e_lane - a.begin] = j < L.num_ops ? fl.sib[1 + j][lane] + extra[1 + j] : 0u;
+        }
     }
 }
 
-// FLAT: every list at size S takes the flat path (pass A); else every
-// embedding takes the general path.  (One kernel holding both paths needs
-// twice the registers of either.)
-template <int S, bool FLAT>
+// FLAT: 0 = every embedding takes the general path; 1 = every list at size
+// S only counts and takes the flat path; 2 = pass A of a size whose lists
+// all take the flat path with per-sibling sizes (pass B takes the general
+// path).  (One kernel holding both paths needs twice the registers of
+// either.)
+template <int S, int FLAT>
 __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
     __shared__ uint32_t sHash[kWavesPerBlock][kHashCap];
-    __shared__ FlatLds sFlat[FLAT ? kWavesPerBlock : 1];
+    __shared__ FlatLds<FLAT == 2> sFlat[FLAT ? kWavesPerBlock : 1];
     static_assert(!FLAT || S >= 2, "the flat path needs an ancestor");
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t w = threadIdx.x >> 6;
@@ -627,9 +722,10 @@ __global__ __launch_bounds__(256) void plan_level_kernel(LevelArgs a) {
                         if (lane == 0) atomicOr(a.err, 1u);
                         continue;
                     }
-                    flat_segment<S>(a, a.lists[sli], b, en, m, idx, wstart, wlen, hk, hc, sFlat[w], acc, lacc);
-                    // flat lists make no children and cache no prefix
-                    if (lane >= b && lane < en) {
+                    flat_segment<S, FLAT == 2>(a, a.lists[sli], b, en, m, idx, wstart, wlen, e, hk, hc, sFlat[w],
+                                               acc, lacc);
+                    // counting lists make no children and cache no prefix
+                    if (FLAT == 1 && lane >= b && lane < en) {
                         if (a.nchild) a.nchild[e - a.begin] = 0;
 #pragma unroll
                         for (int j = 0; j < KZ_MAX_OPS; ++j)
@@ -741,6 +837,38 @@ static bool flat_eligible(const kz_step_list& L, int s, uint32_t cached) {
     return true;
 }
 
+// Whether step list L at size s can take the flat path with per-sibling
+// sizes (pass A of a size that also scatters or caches): every op
+// intersects or subtracts one ancestor set P -- the same for all ops -- and
+// N(m[s-1]); every output is a COUNT or a SCATTER of an op's result, of
+// N(m[s-1]) or of an ancestor set.
+static bool flat_sizes_eligible(const kz_step_list& L, int s) {
+    auto varying = [&](const kz_set_ref& r) { return r.kind == KZ_SET_NBR && int(r.a) == s - 1; };
+    auto shared = [&](const kz_set_ref& r) {
+        return (r.kind == KZ_SET_NBR && int(r.a) < s - 1) || (r.kind == KZ_SET_SLOT && int(r.a) < s);
+    };
+    auto same = [](const kz_set_ref& x, const kz_set_ref& y) { return x.kind == y.kind && x.a == y.a && x.b == y.b; };
+    kz_set_ref P{};
+    for (int j = 0; j < L.num_ops; ++j) {
+        const kz_op& op = L.ops[j];
+        if (op.op != KZ_OP_INTERSECT && op.op != KZ_OP_SUBTRACT) return false;
+        kz_set_ref sh;
+        if (varying(op.lhs) && shared(op.rhs)) sh = op.rhs;
+        else if (shared(op.lhs) && varying(op.rhs)) sh = op.lhs;
+        else return false;
+        if (j == 0) P = sh;
+        else if (!same(P, sh)) return false;
+    }
+    for (int o = 0; o < L.num_outs; ++o) {
+        const kz_out& out = L.outs[o];
+        if (out.kind != KZ_OUT_COUNT && out.kind != KZ_OUT_SCATTER) return false;
+        const kz_set_ref& r = out.src;
+        const bool op_result = r.kind == KZ_SET_SLOT && int(r.a) == s && int(r.b) < L.num_ops;
+        if (!op_result && !varying(r) && !shared(r)) return false;
+    }
+    return true;
+}
+
 struct LevelScratch {
     Scratch nchild, child_off, slot_len[KZ_MAX_OPS], slot_off[KZ_MAX_OPS], cost, bounds;
     Scratch child_parent, child_vertex, slot_data[KZ_MAX_OPS];
@@ -759,7 +887,7 @@ struct Executor {
     LevelScratch ls[KZ_MAX_PATTERN + 1];
     uint32_t cached_mask[KZ_MAX_PATTERN + 1] = {};
     bool has_scatter[KZ_MAX_PATTERN + 1] = {};
-    bool flat_level[KZ_MAX_PATTERN + 1] = {};  // every list at this size takes the flat path
+    int flat_level[KZ_MAX_PATTERN + 1] = {};  // every list at this size takes the flat path: 1 counting, 2 with sizes
     uint32_t grid = 0;
     uint32_t temp_cap = 0;
     kz_run_stats* st = nullptr;
@@ -849,12 +977,14 @@ struct Executor {
         const char* env = std::getenv("KZ_PLAN_FLAT");
         const bool flat_on = !(env && env[0] == '0') && g.max_degree < (1u << 26);
         for (int s = 2; s < K && flat_on; ++s) {
-            bool all = true;
+            bool all = true, all_sizes = true;
             for (int c = 0; c < 256; ++c) {
                 const uint8_t li = p.type_table[s][c];
-                if (li != 0xFF) all &= flat_eligible(p.lists[li], s, cached_mask[s]);
+                if (li == 0xFF) continue;
+                all &= flat_eligible(p.lists[li], s, cached_mask[s]);
+                all_sizes &= flat_sizes_eligible(p.lists[li], s);
             }
-            flat_level[s] = all;
+            flat_level[s] = all ? 1 : (all_sizes ? 2 : 0);
         }
         KZ_HIP(hipMalloc(&d_lists, sizeof(kz_step_list) * std::max<uint32_t>(1, p.num_step_lists)));
         KZ_HIP(hipMalloc(&d_types, sizeof(p.type_table)));
@@ -885,26 +1015,26 @@ struct Executor {
         la.col = g.col;
         la.temp = static_cast<uint32_t*>(temp.ptr);
         la.temp_cap = temp_cap;
-        // a flat size only counts: its pass A is the whole level
-        const bool f = flat_level[la.s];
-        if (f && la.mode != 0) return KZ_OK;
+        // a counting flat size has nothing to write: its pass A is the whole
+        // level; a flat size with sizes writes in pass B on the general path
+        int f = flat_level[la.s];
+        if (f == 1 && la.mode != 0) return KZ_OK;
+        if (la.mode != 0) f = 0;
         const dim3 gd(grid), bd(64 * kWavesPerBlock);
+#define KZ_LEVEL_CASE(S_)                                                                            \
+    case S_:                                                                                         \
+        if (f == 1) hipLaunchKernelGGL((plan_level_kernel<S_, 1>), gd, bd, 0, s, la);                \
+        else if (f == 2) hipLaunchKernelGGL((plan_level_kernel<S_, 2>), gd, bd, 0, s, la);           \
+        else hipLaunchKernelGGL((plan_level_kernel<S_, 0>), gd, bd, 0, s, la);                       \
+        break;
         switch (la.s) {
-            case 1: hipLaunchKernelGGL((plan_level_kernel<1, false>), gd, bd, 0, s, la); break;
-            case 2:
-                if (f) hipLaunchKernelGGL((plan_level_kernel<2, true>), gd, bd, 0, s, la);
-                else hipLaunchKernelGGL((plan_level_kernel<2, false>), gd, bd, 0, s, la);
-                break;
-            case 3:
-                if (f) hipLaunchKernelGGL((plan_level_kernel<3, true>), gd, bd, 0, s, la);
-                else hipLaunchKernelGGL((plan_level_kernel<3, false>), gd, bd, 0, s, la);
-                break;
-            case 4:
-                if (f) hipLaunchKernelGGL((plan_level_kernel<4, true>), gd, bd, 0, s, la);
-                else hipLaunchKernelGGL((plan_level_kernel<4, false>), gd, bd, 0, s, la);
-                break;
+            case 1: hipLaunchKernelGGL((plan_level_kernel<1, 0>), gd, bd, 0, s, la); break;
+            KZ_LEVEL_CASE(2)
+            KZ_LEVEL_CASE(3)
+            KZ_LEVEL_CASE(4)
             default: KZ_FAIL(KZ_ERR_INVALID, "embedding size %d outside 1..%d", la.s, KZ_MAX_PATTERN - 1);
         }
+#undef KZ_LEVEL_CASE
         KZ_HIP(hipGetLastError());
         if (st) st->launches++;
         return KZ_OK;
