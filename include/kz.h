@@ -344,6 +344,17 @@ typedef struct kz_fsm_candidate {
 int kz_fsm_mine(kz_graph g, uint32_t threshold, kz_fsm_stats* stats, kz_fsm_candidate* cands, uint32_t cap,
                 uint32_t* ncand);
 
+/* The minimum-image domains of ONE labelled pattern of a kz_fsm_mine level
+ * (KZ_FSM_EDGE .. KZ_FSM_THREE_STAR; labels in that level's position order,
+ * see the KZ_FSM_* comments): sizes[i] = how many distinct vertices take
+ * position i in some embedding (sizes of unused positions are 0).  This is
+ * what one run() of kautomine/three_fsm.cc collects in its
+ * DistributedPatternDomain (engine.h:405-580) through insert() calls from
+ * apply() (three_fsm.cc:52-250), the support being the smallest size
+ * (get_support, engine.h:486-560); libkhuzdul_amd.so calls it for the
+ * unchanged app.  Needs kz_graph_set_labels. */
+int kz_fsm_domains(kz_graph g, uint32_t level, const uint32_t* labels, uint32_t* sizes);
+
 /* ---- Serving --------------------------------------------------------------
  * A server runs queries on resident graphs from a worker thread.  Queries
  * are admitted in arrival order while the device memory of the prepared

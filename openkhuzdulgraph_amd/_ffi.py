@@ -251,6 +251,7 @@ _SIGS = {
     "kz_clique_workload": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "kz_motif_census": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(kz_run_stats)]),
     "kz_graph_set_labels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "kz_fsm_domains": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "kz_fsm_mine": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(kz_fsm_stats), C.POINTER(kz_fsm_candidate),
                               C.c_uint32, C.POINTER(C.c_uint32)]),
     "kz_server_start": (C.c_int, [C.POINTER(kz_server_cfg), C.POINTER(C.c_void_p)]),

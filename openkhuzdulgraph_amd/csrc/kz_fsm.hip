@@ -394,6 +394,84 @@ int fsm_set_labels(DeviceGraph& d, const uint32_t* h_lab) {
     return KZ_OK;
 }
 
+// The domain kernels of one candidate of a level (1 edge .. 5 three-star)
+// with labels l[], into its 4 bitmaps bm[4 * W].
+static void launch_domains(const DeviceGraph& d, const FsmState& F, int level, const uint32_t* l, uint32_t* bm,
+                           uint32_t W, uint32_t grid, hipStream_t s) {
+    FsmArgs a{d.rowptr, F.lcol, F.loff, F.L + 1, nullptr, 0, {0, 0, 0, 0}, bm, W};
+    const int size = level == 1 ? 2 : (level <= 3 ? 3 : 4);
+    for (int i = 0; i < size; ++i) a.l[i] = l[i];
+    // roots: the vertices of the first label; the three-star and the
+    // four-chain's second pass start from their center / v1
+    auto roots = [&](uint32_t lab) {
+        a.roots = F.vlist + F.voff[lab];
+        a.nroots = F.voff[lab + 1] - F.voff[lab];
+    };
+    switch (level) {
+        case 1:
+            roots(l[0]);
+            if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<1>, dim3(grid), dim3(256), 0, s, a);
+            break;
+        case 2:
+            roots(l[0]);
+            if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<2>, dim3(grid), dim3(256), 0, s, a);
+            break;
+        case 3:
+            roots(l[0]);
+            if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<3>, dim3(grid), dim3(256), 0, s, a);
+            break;
+        case 4:
+            roots(l[0]);
+            if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<4>, dim3(grid), dim3(256), 0, s, a);
+            roots(l[1]);
+            if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<5>, dim3(grid), dim3(256), 0, s, a);
+            break;
+        default:
+            roots(l[1]);
+            if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<6>, dim3(grid), dim3(256), 0, s, a);
+            break;
+    }
+}
+
+int fsm_domains(DeviceGraph& d, int level, const uint32_t* labels, uint32_t* sizes) {
+    if (int e = ensure_ready()) return e;
+    if (!d.fsm) KZ_FAIL(KZ_ERR_STATE, "the graph has no labels (kz_graph_set_labels)");
+    if (level < KZ_FSM_EDGE || level > KZ_FSM_THREE_STAR) KZ_FAIL(KZ_ERR_INVALID, "level %d is not a pattern level", level);
+    FsmState& F = *static_cast<FsmState*>(d.fsm.get());
+    const int size = level == 1 ? 2 : (level <= 3 ? 3 : 4);
+    for (int i = 0; i < size; ++i)
+        if (labels[i] >= F.L) {
+            // a label no vertex carries: every domain is empty
+            for (int j = 0; j < 4; ++j) sizes[j] = 0;
+            return KZ_OK;
+        }
+    hipStream_t s = rt().stream;
+    const uint32_t W = std::max<uint32_t>(1, (F.n + 31) / 32);
+    uint32_t *bm = nullptr, *sz = nullptr;
+    if (hipMalloc(&bm, 16 * size_t(W)) != hipSuccess || hipMalloc(&sz, 16) != hipSuccess) {
+        (void)hipGetLastError();
+        hipFree(bm);
+        KZ_FAIL(KZ_ERR_NOMEM, "domain bitmaps of %u vertices", F.n);
+    }
+    struct Free {
+        uint32_t *a, *b;
+        ~Free() {
+            hipFree(a);
+            hipFree(b);
+        }
+    } free_on_return{bm, sz};
+    KZ_HIP(hipMemsetAsync(bm, 0, 16 * size_t(W), s));
+    KZ_HIP(hipMemsetAsync(sz, 0, 16, s));
+    launch_domains(d, F, level, labels, bm, W, uint32_t(rt().num_cus) * 8, s);
+    const uint32_t pgrid = std::min<uint32_t>((W + 255) / 256, 64);
+    hipLaunchKernelGGL(fsm_popcount_kernel, dim3(pgrid, 4), dim3(256), 0, s, bm, W, sz);
+    KZ_HIP(hipGetLastError());
+    KZ_HIP(hipMemcpyAsync(sizes, sz, 16, hipMemcpyDeviceToHost, s));
+    KZ_HIP(hipStreamSynchronize(s));
+    for (int j = size; j < 4; ++j) sizes[j] = 0;
+    return KZ_OK;
+}
+
 int fsm_mine(DeviceGraph& d, uint32_t threshold, kz_fsm_stats* st, kz_fsm_candidate* out, uint32_t cap,
              uint32_t* ncand) {
     if (int e = ensure_ready()) return e;
@@ -454,38 +532,9 @@ int fsm_mine(DeviceGraph& d, uint32_t threshold, kz_fsm_stats* st, kz_fsm_candid
             KZ_HIP(hipMemsetAsync(sz, 0, 16 * size_t(nb), s));
             for (uint32_t c = 0; c < nb; ++c) {
                 const Pat& p = cs[c0 + c].canon;
-                FsmArgs a{d.rowptr, F.lcol, F.loff, L + 1, nullptr, 0, {0, 0, 0, 0}, bm + size_t(c) * 4 * W, W};
-                for (int i = 0; i < p.size; ++i) a.l[i] = p.l[i];
-                // roots: the vertices of the first label; the three-star and
-                // the four-chain's second pass start from their center / v1
-                auto roots = [&](uint32_t l) {
-                    a.roots = F.vlist + F.voff[l];
-                    a.nroots = F.voff[l + 1] - F.voff[l];
-                };
-                switch (level) {
-                    case 1:
-                        roots(p.l[0]);
-                        if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<1>, dim3(grid), dim3(256), 0, s, a);
-                        break;
-                    case 2:
-                        roots(p.l[0]);
-                        if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<2>, dim3(grid), dim3(256), 0, s, a);
-                        break;
-                    case 3:
-                        roots(p.l[0]);
-                        if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<3>, dim3(grid), dim3(256), 0, s, a);
-                        break;
-                    case 4:
-                        roots(p.l[0]);
-                        if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<4>, dim3(grid), dim3(256), 0, s, a);
-                        roots(p.l[1]);
-                        if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<5>, dim3(grid), dim3(256), 0, s, a);
-                        break;
-                    default:
-                        roots(p.l[1]);
-                        if (a.nroots) hipLaunchKernelGGL(fsm_domain_kernel<6>, dim3(grid), dim3(256), 0, s, a);
-                        break;
-                }
+                uint32_t l[4] = {0, 0, 0, 0};
+                for (int i = 0; i < p.size; ++i) l[i] = p.l[i];
+                launch_domains(d, F, level, l, bm + size_t(c) * 4 * W, W, grid, s);
             }
             const uint32_t pgrid = std::min<uint32_t>((W + 255) / 256, 64);
             hipLaunchKernelGGL(fsm_popcount_kernel, dim3(pgrid, 4 * nb), dim3(256), 0, s, bm, W, sz);
@@ -596,6 +645,11 @@ extern "C" {
 int kz_graph_set_labels(kz_graph g, const uint32_t* labels) {
     if (!g || (!labels && g->g.n)) KZ_FAIL(KZ_ERR_INVALID, "null argument");
     return fsm_set_labels(g->g, labels);
+}
+
+int kz_fsm_domains(kz_graph g, uint32_t level, const uint32_t* labels, uint32_t* sizes) {
+    if (!g || !labels || !sizes) KZ_FAIL(KZ_ERR_INVALID, "null argument");
+    return fsm_domains(g->g, int(level), labels, sizes);
 }
 
 int kz_fsm_mine(kz_graph g, uint32_t threshold, kz_fsm_stats* stats, kz_fsm_candidate* cands, uint32_t cap,

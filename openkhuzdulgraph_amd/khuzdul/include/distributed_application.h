@@ -21,6 +21,8 @@
 #include "graph.h"
 #include "graph_loader.h"
 
+struct kz_csr;  // include/kz.h
+
 namespace Khuzdul {
 
 typedef HashBasedNumaAwareDistributedCSRGraph DistGraph;
@@ -39,11 +41,14 @@ class DistributedApplication {
     void* device_graph_;  // kz_graph
     void* paged_graph_;   // kz_paged: lists in host memory (beyond the resident budget)
     bool probe_only_;
+    bool labelled_;  // load_vertex_labels: run() counts labelled patterns
     typedef void (*AddFn)(void*, uint64_t);
     std::vector<void*> aggregators_;  // in allocation order
     std::vector<AddFn> aggregator_add_;
 
     void register_aggregator(void* a, AddFn add);
+    void set_host_labels(const kz_csr& csr);
+    void run_labelled(bool restrict_starting_vertex_label, LabelId starting_vertex_label);
     void unregister_aggregator(void* a);
 
    public:

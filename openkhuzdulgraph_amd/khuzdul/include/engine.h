@@ -24,6 +24,7 @@
 #include <condition_variable>
 #include <functional>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -119,6 +120,99 @@ class Aggregator {
     ~Aggregator() {
         for (int s = 0; s < num_sockets_; ++s)
             for (int t = 0; t < num_threads_; ++t) free(local_reducers_[s][t]);
+    }
+};
+
+// Labelled patterns (include/engine.h:231-301 of the reference): the same
+// fields and equality; the set of patterns keeps its interface (insert,
+// is_exist, clear; the table size argument is accepted and not needed).
+struct LabeledPattern {
+    LabelId labels[MAX_PATTERN_SIZE];
+    int size;
+    bool operator==(const LabeledPattern& other) const {
+        if (size != other.size) return false;
+        for (int i = 0; i < size; ++i)
+            if (labels[i] != other.labels[i]) return false;
+        return true;
+    }
+} __attribute__((packed));
+
+class LabeledPatternHashTable {
+   private:
+    std::set<std::vector<LabelId>> entries_;
+    static std::vector<LabelId> key(const LabeledPattern& p) {
+        std::vector<LabelId> k(1, LabelId(p.size));
+        for (int i = 0; i < p.size; ++i) k.push_back(p.labels[i]);
+        return k;
+    }
+
+   public:
+    explicit LabeledPatternHashTable(size_t table_size = 1000000) { (void)table_size; }
+    void clear() { entries_.clear(); }
+    void insert(const LabeledPattern p) { entries_.insert(key(p)); }
+    bool is_exist(const LabeledPattern p) { return entries_.count(key(p)) != 0; }
+};
+
+// Minimum-image domains of one labelled pattern (include/engine.h:405-580 of
+// the reference: the same constructor, clear(), insert() and get_support()).
+// A vertex joins domain i when apply() inserts it at pattern vertex i with
+// enable set; the support is the smallest domain, at most the vertex count.
+// On this engine apply() runs only while DistributedApplication::run()
+// probes it: run() computes the domains on the GPU (kz_fsm_domains) and
+// stores their sizes here.  An insert() outside a run (direct use of the
+// class) goes to host bitmaps.
+class DistributedPatternDomain {
+   private:
+    int max_pattern_size_;
+    VertexId num_vertices_;
+    bool from_engine_ = false;
+    VertexId engine_sizes_[MAX_PATTERN_SIZE] = {};
+    std::vector<uint64_t> bits_[MAX_PATTERN_SIZE];
+
+   public:
+    DistributedPatternDomain(int max_pattern_size, int num_threads, int num_sockets, VertexId num_vertices)
+        : max_pattern_size_(max_pattern_size), num_vertices_(num_vertices) {
+        (void)num_threads;
+        (void)num_sockets;
+        assert(max_pattern_size_ >= 1 && max_pattern_size_ <= MAX_PATTERN_SIZE);
+    }
+    void clear() {
+        from_engine_ = false;
+        for (int i = 0; i < MAX_PATTERN_SIZE; ++i) {
+            engine_sizes_[i] = 0;
+            bits_[i].clear();
+        }
+    }
+    void insert(Context context, int pattern_vertex, VertexId input_graph_vertex, bool enable) {
+        (void)context;
+        if (__builtin_expect(kzk::g_hooks != nullptr, 0)) {
+            kzk::g_hooks->on_domain_insert(this, pattern_vertex, input_graph_vertex, enable);
+            return;
+        }
+        assert(pattern_vertex >= 0 && pattern_vertex < max_pattern_size_);
+        if (!enable) return;
+        std::vector<uint64_t>& b = bits_[pattern_vertex];
+        if (b.empty()) b.assign((size_t(num_vertices_) + 63) / 64, 0);
+        b[input_graph_vertex >> 6] |= 1ull << (input_graph_vertex & 63);
+    }
+    // the engine's domains for this run (run() calls it)
+    void set_domain_sizes(const VertexId* sizes, int n) {
+        clear();
+        from_engine_ = true;
+        for (int i = 0; i < n && i < MAX_PATTERN_SIZE; ++i) engine_sizes_[i] = sizes[i];
+    }
+    VertexId get_support(int pattern_size) {
+        VertexId support = num_vertices_;
+        for (int i = 0; i < pattern_size && i < MAX_PATTERN_SIZE; ++i) {
+            VertexId d = 0;
+            if (from_engine_) {
+                d = engine_sizes_[i];
+            } else {
+                for (uint64_t w : bits_[i]) d += VertexId(__builtin_popcountll(w));
+            }
+            support = d < support ? d : support;
+        }
+        return support;
     }
 };
 

@@ -16,6 +16,13 @@ struct ProbeHooks {
     virtual void on_trim(const VertexId* list, VertexId n) = 0;
     virtual void on_add(const void* aggregator, uint64_t delta) = 0;
     virtual void on_create_cached(VertexId* start, VertexId* obj, VertexId size) = 0;
+    // DistributedPatternDomain::insert from apply() (labelled mining)
+    virtual void on_domain_insert(const void* domain, int pattern_vertex, VertexId v, bool enable) {
+        (void)domain;
+        (void)pattern_vertex;
+        (void)v;
+        (void)enable;
+    }
     virtual ~ProbeHooks() {}
 };
 

@@ -155,10 +155,7 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
         std::fprintf(stderr, "pattern size %d exceeds MAX_PATTERN_SIZE %d\n", max_pattern_size, MAX_PATTERN_SIZE);
         std::exit(-1);
     }
-    if (load_vertex_labels) {
-        std::fprintf(stderr, "vertex labels are not supported by this engine (labelled FSM is out of scope)\n");
-        std::exit(-1);
-    }
+    labelled_ = load_vertex_labels;
     max_pattern_size_ = max_pattern_size;
     max_depth_ = max_pattern_size - 2;
     apply_fun_ = apply_fun;
@@ -176,6 +173,18 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
     paged_graph_ = nullptr;
     if (probe_only_) {
         std::printf("KZ_PROBE_ONLY: plans are extracted and dumped, nothing is counted\n");
+        if (labelled_) {
+            // the labelled probe needs the label count (the .dgraph is read
+            // on the host; no GPU)
+            kz_csr csr;
+            std::memset(&csr, 0, sizeof(csr));
+            die_kz(kz_dgraph_load(graph_path.c_str(), &csr), "kz_dgraph_load");
+            graph_.num_vertices_ = csr.num_vertices;
+            graph_.num_edges_ = csr.num_entries;
+            graph_.max_degree_ = csr.max_degree;
+            set_host_labels(csr);
+            kz_csr_release(&csr);
+        }
         return;
     }
     const char* dev = std::getenv("KZ_DEVICE");
@@ -194,6 +203,10 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
     graph_.num_edges_ = csr.num_entries;
     graph_.max_degree_ = csr.max_degree;
     graph_.oriented_ = csr.oriented != 0;
+    if (labelled_) {
+        set_host_labels(csr);
+        std::printf("    Number of labels: %u\n", graph_.num_labels_);
+    }
     // KZ_RESIDENT_BYTES: HBM budget for the neighbour lists.  A graph whose
     // lists exceed it keeps them in host memory and is counted in root
     // batches (kz_paged_*), with cache_degree_threshold and
@@ -202,6 +215,10 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
     const char* rb = std::getenv("KZ_RESIDENT_BYTES");
     const uint64_t resident = rb ? std::strtoull(rb, nullptr, 10) : 0;
     if (resident && 4 * csr.num_entries > resident) {
+        if (labelled_) {
+            std::fprintf(stderr, "a labelled graph must fit KZ_RESIDENT_BYTES\n");
+            std::exit(-1);
+        }
         kz_csr ori;
         std::memset(&ori, 0, sizeof(ori));
         const kz_csr* src = &csr;
@@ -234,6 +251,130 @@ DistributedApplication::DistributedApplication(int num_threads, std::string grap
     t += get_time();
     LocalPerformanceMetric::load_graph_time = t;
     std::printf("It takes %.3f seconds to load the graph into HBM.\n", t);
+}
+
+// Labels of a .dgraph read with load_vertex_labels: the label count and
+// each label's vertices (graph.h:108-114 of the reference).  The device
+// copy is attached by kz_graph_upload (a kz_csr carrying labels).
+void DistributedApplication::set_host_labels(const kz_csr& csr) {
+    if (!csr.has_labels || (!csr.labels && csr.num_vertices)) {
+        std::fprintf(stderr, "load_vertex_labels: the graph file carries no vertex labels\n");
+        std::exit(-1);
+    }
+    LabelId L = 0;
+    for (VertexId v = 0; v < csr.num_vertices; ++v) L = std::max<LabelId>(L, csr.labels[v] + 1);
+    graph_.num_labels_ = L;
+    graph_.labeled_vertices_.assign(L, std::vector<VertexId>());
+    for (VertexId v = 0; v < csr.num_vertices; ++v) graph_.labeled_vertices_[csr.labels[v]].push_back(v);
+}
+
+// One run() of a labelled app: apply() is probed into a labelled pattern
+// (probe.cpp), its minimum-image domains come from the GPU
+// (kz_fsm_domains, on the kz_fsm_mine level of the pattern's shape) and go
+// into the app's DistributedPatternDomain, whose get_support() the app then
+// reads (three_fsm.cc:572-587).  Shapes: edge, wedge, triangle, four-chain,
+// three-star (every pattern kautomine/three_fsm.cc mines).
+void DistributedApplication::run_labelled(bool restrict_starting_vertex_label, LabelId starting_vertex_label) {
+    kzk::LabeledPlan lp;
+    std::string err;
+    double t = -get_time();
+    if (kzk::extract_labeled_pattern(apply_fun_, max_pattern_size_, graph_.num_labels_, restrict_starting_vertex_label,
+                                     starting_vertex_label, &lp, &err) != 0) {
+        std::fprintf(stderr, "cannot turn the labelled apply() into a GPU pattern: %s\n", err.c_str());
+        std::exit(-1);
+    }
+    t += get_time();
+    LocalPerformanceMetric::probe_time_ms += t * 1e3;
+    if (lp.empty || !lp.domain) return;  // no embedding: the app's cleared domain gives support 0
+    // the kz_fsm_mine level and its position order (the KZ_FSM_* comments)
+    const int k = lp.size;
+    int deg[MAX_PATTERN_SIZE] = {0};
+    int edges = 0;
+    for (int i = 0; i < k; ++i) {
+        deg[i] = __builtin_popcount(lp.adj[i]);
+        edges += deg[i];
+    }
+    edges /= 2;
+    int level = -1;
+    int order[4] = {0, 1, 2, 3};  // level position -> pattern vertex
+    auto other = [&](int i, int not1, int not2) {
+        for (int j = 0; j < k; ++j)
+            if ((lp.adj[i] >> j & 1u) && j != not1 && j != not2) return j;
+        return -1;
+    };
+    if (k == 2 && edges == 1) {
+        level = KZ_FSM_EDGE;
+    } else if (k == 3 && edges == 3) {
+        level = KZ_FSM_TRIANGLE;
+    } else if (k == 3 && edges == 2) {
+        level = KZ_FSM_WEDGE;  // center, leaf, leaf
+        const int c = deg[0] == 2 ? 0 : (deg[1] == 2 ? 1 : 2);
+        order[0] = c;
+        order[1] = other(c, -1, -1);
+        order[2] = other(c, order[1], -1);
+    } else if (k == 4 && edges == 3) {
+        int c = -1, leaves = 0;
+        for (int i = 0; i < 4; ++i) {
+            if (deg[i] == 3) c = i;
+            leaves += deg[i] == 1;
+        }
+        if (c >= 0) {
+            level = KZ_FSM_THREE_STAR;  // leaf, center, leaf, leaf
+            order[1] = c;
+            int o = 0;
+            for (int i = 0; i < 4; ++i)
+                if (i != c) order[o == 0 ? 0 : o + 1] = i, ++o;
+        } else if (leaves == 2) {
+            level = KZ_FSM_FOUR_CHAIN;  // path order[2] - order[0] - order[1] - order[3]
+            int p = -1;
+            for (int i = 0; i < 4 && p < 0; ++i)
+                if (deg[i] == 2) p = i;
+            const int n1 = other(p, -1, -1), n2 = other(p, n1, -1);
+            const int q = deg[n1] == 2 ? n1 : n2;  // p's inner neighbour
+            order[0] = p;
+            order[1] = q;
+            order[2] = q == n1 ? n2 : n1;
+            order[3] = other(q, p, -1);
+        }
+    }
+    if (level < 0) {
+        std::fprintf(stderr, "labelled pattern of %d vertices and %d edges: only edges, wedges, triangles, "
+                             "four-chains and three-stars are counted on the GPU\n", k, edges);
+        std::exit(-1);
+    }
+    uint32_t labels[4] = {0, 0, 0, 0}, sizes[4] = {0, 0, 0, 0};
+    for (int i = 0; i < k; ++i) labels[i] = lp.labels[order[i]];
+    if (const char* tr = std::getenv("KZ_LABELLED_TRACE")) {
+        // one line per run: the recovered pattern and its kz_fsm_mine level
+        if (FILE* f = std::fopen(tr, "a")) {
+            std::fprintf(f, "level %d labels", level);
+            for (int i = 0; i < k; ++i) std::fprintf(f, " %u", labels[i]);
+            std::fprintf(f, " adj");
+            for (int i = 0; i < k; ++i) std::fprintf(f, " %u", lp.adj[i]);
+            std::fprintf(f, "\n");
+            std::fclose(f);
+        }
+    }
+    if (probe_only_) {
+        // nothing is counted: every pattern is reported with the largest
+        // support (the vertex count), so that every level's candidates are
+        // probed
+        VertexId dom[MAX_PATTERN_SIZE];
+        for (int i = 0; i < MAX_PATTERN_SIZE; ++i) dom[i] = graph_.num_vertices_;
+        static_cast<Khuzdul::DistributedPatternDomain*>(lp.domain)->set_domain_sizes(dom, k);
+        return;
+    }
+    kz_graph g = static_cast<kz_graph>(device_graph_);
+    double tg = -get_time();
+    die_kz(kz_fsm_domains(g, uint32_t(level), labels, sizes), "kz_fsm_domains");
+    tg += get_time();
+    VertexId dom[MAX_PATTERN_SIZE] = {0};
+    for (int i = 0; i < k; ++i) dom[order[i]] = sizes[i];
+    static_cast<Khuzdul::DistributedPatternDomain*>(lp.domain)->set_domain_sizes(dom, k);
+    LocalPerformanceMetric::gpu_time_ms += tg * 1e3;
+    LocalPerformanceMetric::num_runs++;
+    LocalPerformanceMetric::num_launches++;
+    LocalPerformanceMetric::last_method = "labelled pattern domains (kz_fsm_domains)";
 }
 
 DistributedApplication::~DistributedApplication() {
@@ -277,9 +418,12 @@ static bool same_plan(const kz_plan_desc& a, const kz_plan_desc& b) {
 
 void DistributedApplication::run(bool disable_output, bool restrict_starting_vertex_label,
                                  LabelId starting_vertex_label) {
-    (void)starting_vertex_label;
+    if (labelled_) {
+        run_labelled(restrict_starting_vertex_label, starting_vertex_label);
+        return;
+    }
     if (restrict_starting_vertex_label) {
-        std::fprintf(stderr, "label-restricted runs are not supported by this engine\n");
+        std::fprintf(stderr, "label-restricted runs need the graph's labels (load_vertex_labels)\n");
         std::exit(-1);
     }
     kz_plan_desc plan;

@@ -124,7 +124,13 @@ uint32_t code_of(const std::vector<int>& ranks) {
     return code;
 }
 
-class Prober final : public ProbeHooks, public Khuzdul::AbstractEmbeddingExplorationEngine {
+// Receiver of the VertexSet operations apply() issues while probed.
+struct OpSink {
+    virtual void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) = 0;
+    virtual ~OpSink() {}
+};
+
+class Prober final : public ProbeHooks, public Khuzdul::AbstractEmbeddingExplorationEngine, public OpSink {
    public:
     Prober(Khuzdul::ApplyFunction fn, int K, const std::vector<void*>& aggs) : fn_(fn), K_(K), aggs_(aggs) {}
 
@@ -190,7 +196,7 @@ class Prober final : public ProbeHooks, public Khuzdul::AbstractEmbeddingExplora
 
     // VertexSet operations while probing (called from the out-of-line
     // VertexSet methods in khuzdul_api.cpp).
-    void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) {
+    void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) override {
         Event e;
         e.type = Event::OP;
         e.op = op;
@@ -483,7 +489,268 @@ void Prober::extract(kz_plan_desc* plan) {
     for (size_t i = 0; i < lists.size(); ++i) plan->lists[i] = lists[i];
 }
 
-Prober* g_prober = nullptr;
+// ------------------------------------------------------------ labelled apps
+// A labelled app (kautomine/three_fsm.cc) runs once per candidate pattern:
+// its apply() (:52-250) follows global pattern labels, extends an embedding
+// by neighbours whose label matches the next pattern vertex, and at the
+// last one inserts every matched vertex into a DistributedPatternDomain
+// with `enable` = the candidate's label matches (and it repeats no matched
+// vertex).  The probe recovers that pattern: at each size s it runs apply()
+// with every neighbour list holding one fresh vertex x, once per label of
+// x; exactly one label must lead to a scatter of x or to an enabled insert
+// of x.  The list the candidate loop read gives x's neighbours in the
+// pattern (N(m_j): j; an intersection: every operand's vertex).  A run with
+// the list holding a matched vertex m_j that could also be x (same label,
+// adjacent to all of x's neighbours) must be rejected.  Inserts end the
+// pattern: every position is inserted with its matched vertex.
+class LabelProber final : public ProbeHooks, public Khuzdul::AbstractEmbeddingExplorationEngine, public OpSink {
+   public:
+    LabelProber(Khuzdul::ApplyFunction fn, int K, LabelId L) : fn_(fn), K_(K), L_(L) {}
+
+    void extract(bool root_known, LabelId root_label, LabeledPlan* out);
+
+    // ---- hooks
+    void on_read(const VertexId* list, VertexId n) override {
+        (void)n;
+        Ev e;
+        e.type = Ev::READ;
+        e.src = prov_of(list);
+        log_.push_back(e);
+    }
+    void on_trim(const VertexId* list, VertexId n) override { on_read(list, n); }
+    void on_add(const void*, uint64_t) override { fail("a labelled apply() adds to an aggregator"); }
+    void on_create_cached(VertexId*, VertexId*, VertexId) override {}
+    void on_domain_insert(const void* domain, int pattern_vertex, VertexId v, bool enable) override {
+        Ev e;
+        e.type = Ev::INSERT;
+        e.domain = domain;
+        e.idx = pattern_vertex;
+        e.v = v;
+        e.enable = enable;
+        log_.push_back(e);
+    }
+    void scatter(const VertexId new_v, const bool, const uint32_t, Khuzdul::ExtendableEmbedding&, const Context,
+                 const bool) override {
+        Ev e;
+        e.type = Ev::SCATTER;
+        e.v = new_v;
+        log_.push_back(e);
+    }
+    void* alloc_thread_local_scratchpad_buffer(const Context&, const size_t& size) override {
+        const size_t words = (size + 3) / 4 + 8;
+        scratch_.emplace_back(new VertexId[words]());
+        regions_.push_back(Region{scratch_.back().get(), words, false, 0});
+        return scratch_.back().get();
+    }
+    void clear_thread_local_scratchpad_buffer(const Context&) override {}
+    LabelId get_vertex_label(const VertexId v) override {
+        auto it = label_.find(v);
+        if (it == label_.end()) fail("apply() reads the label of a vertex the probe did not place");
+        return it->second;
+    }
+    void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) override {
+        if (op != KZ_OP_INTERSECT) fail("a labelled apply() subtracts sets (induced patterns are not supported)");
+        Ev e;
+        e.type = Ev::OP;
+        e.lhs = prov_of(lhs->vertices_list_);
+        e.rhs = prov_of(rhs->vertices_list_);
+        const int idx = int(ops_.size());
+        ops_.push_back(e);
+        log_.push_back(e);
+        const Region* r = nullptr;
+        for (auto it = regions_.rbegin(); it != regions_.rend() && !r; ++it)
+            if (result->vertices_list_ >= it->start && result->vertices_list_ < it->start + it->cap) r = &*it;
+        if (!r) fail("a set operation writes outside the scratchpad");
+        // every list holds the same content, so the intersection is that content
+        if (content_.size() > r->cap - size_t(result->vertices_list_ - r->start))
+            fail("probe content does not fit a result buffer");
+        std::copy(content_.begin(), content_.end(), result->vertices_list_);
+        result->num_vertices_ = VertexId(content_.size());
+        Prov pv;
+        pv.kind = KZ_SET_SLOT;
+        pv.b = idx;
+        prov_[result->vertices_list_] = pv;
+    }
+
+   private:
+    struct Ev {
+        enum Type { READ, OP, SCATTER, INSERT } type;
+        Prov src, lhs, rhs;
+        const void* domain = nullptr;
+        int idx = 0;
+        VertexId v = 0;
+        bool enable = false;
+    };
+    [[noreturn]] void fail(const std::string& msg) { throw std::runtime_error(msg); }
+    Prov prov_of(const VertexId* p) {
+        auto it = prov_.find(p);
+        if (it == prov_.end()) fail("apply() used a vertex list the probe cannot attribute");
+        return it->second;
+    }
+    // Pattern vertices whose neighbour lists feed the set `p`.
+    uint32_t sources(const Prov& p) {
+        if (p.kind == KZ_SET_NBR) return 1u << p.a;
+        if (p.kind == KZ_SET_SLOT && p.b < int(ops_.size())) return sources(ops_[p.b].lhs) | sources(ops_[p.b].rhs);
+        fail("a candidate loop reads a set the probe cannot attribute");
+    }
+    std::vector<Ev> run(const std::vector<VertexId>& m, const std::vector<VertexId>& content) {
+        const int s = int(m.size());
+        prov_.clear();
+        regions_.clear();
+        scratch_.clear();
+        ops_.clear();
+        log_.clear();
+        nbr_bufs_.assign(size_t(s), content);
+        cached_.assign(size_t(s), std::vector<VertexId>(1 << 12, 0));
+        content_ = content;
+        Khuzdul::ExtendableEmbedding e;
+        std::memset(&e, 0, sizeof(e));
+        e.size = s;
+        for (int j = 0; j < s; ++j) {
+            e.matched_vertices[j] = m[j];
+            e.matched_vertices_nbrs_ptx[j] = nbr_bufs_[j].data();
+            e.matched_vertices_num_nbrs[j] = VertexId(content.size());
+            e.cached_objects[j] = cached_[j].data();
+            regions_.push_back(Region{cached_[j].data(), cached_[j].size(), true, j + 1});
+            Prov pv;
+            pv.kind = KZ_SET_NBR;
+            pv.a = j;
+            prov_[nbr_bufs_[j].data()] = pv;
+        }
+        Context ctx;
+        std::memset(&ctx, 0, sizeof(ctx));
+        ctx.depth = s - 1;
+        fn_(e, ctx, this);
+        return log_;
+    }
+
+    Khuzdul::ApplyFunction fn_;
+    int K_;
+    LabelId L_;
+    std::map<VertexId, LabelId> label_;
+    std::map<const VertexId*, Prov> prov_;
+    std::vector<Region> regions_;
+    std::vector<std::unique_ptr<VertexId[]>> scratch_;
+    std::vector<std::vector<VertexId>> nbr_bufs_, cached_;
+    std::vector<Ev> ops_, log_;
+    std::vector<VertexId> content_;
+};
+
+void LabelProber::extract(bool root_known, LabelId root_label, LabeledPlan* out) {
+    *out = LabeledPlan();
+    auto vid = [](int i) { return kBase + kGap * VertexId(i + 1); };
+    std::vector<VertexId> m = {vid(0)};
+    std::vector<LabelId> lab = {root_label};
+    std::vector<uint32_t> adj = {0};
+    if (!root_known) {
+        // the root's label: the one for which apply() does anything
+        int found = -1;
+        for (LabelId l = 0; l < L_; ++l) {
+            bool any = false;
+            for (LabelId c = 0; c < L_ && !any; ++c) {
+                label_.clear();
+                label_[m[0]] = l;
+                label_[vid(1)] = c;
+                const std::vector<Ev> ev = run(m, {vid(1)});
+                for (const Ev& e : ev) any |= e.type != Ev::READ;
+            }
+            if (any) {
+                if (found >= 0) fail("apply() accepts roots of more than one label: run() needs the starting label");
+                found = int(l);
+            }
+        }
+        if (found < 0) {
+            out->empty = true;
+            return;
+        }
+        lab[0] = LabelId(found);
+    }
+    for (int s = 1;; ++s) {
+        if (s >= MAX_PATTERN_SIZE) fail("a labelled pattern larger than MAX_PATTERN_SIZE");
+        const VertexId x = vid(s);
+        label_.clear();
+        for (int j = 0; j < s; ++j) label_[m[j]] = lab[j];
+        int acc = -1;
+        bool inserts = false, scatters = false;
+        uint32_t srcs = 0;
+        std::vector<Ev> acc_ev;
+        for (LabelId l = 0; l < L_; ++l) {
+            label_[x] = l;
+            const std::vector<Ev> ev = run(m, {x});
+            bool takes = false;
+            Prov last_read;
+            bool have_read = false;
+            for (const Ev& e : ev) {
+                if (e.type == Ev::READ) {
+                    last_read = e.src;
+                    have_read = true;
+                } else if (e.type == Ev::SCATTER && e.v == x) {
+                    scatters = true;
+                    if (!have_read) fail("a scatter without a preceding candidate read");
+                    takes = true;
+                    srcs = sources(last_read);
+                } else if (e.type == Ev::INSERT) {
+                    inserts = true;
+                    if (e.idx == s && e.v == x && e.enable) {
+                        if (!have_read) fail("an insert without a preceding candidate read");
+                        takes = true;
+                        srcs = sources(last_read);
+                    }
+                }
+            }
+            if (takes) {
+                if (acc >= 0) fail("apply() takes a candidate of more than one label at one pattern vertex");
+                acc = int(l);
+                acc_ev = ev;
+            }
+        }
+        if (inserts && scatters) fail("apply() both scatters and inserts at one size");
+        if (acc < 0) {
+            out->empty = true;  // no label extends the pattern here: no embedding
+            return;
+        }
+        // a matched vertex that could also be the candidate must be refused
+        for (int j = 0; j < s; ++j) {
+            if (lab[j] != LabelId(acc) || (srcs & ~adj[j]) != 0) continue;
+            const std::vector<Ev> ev = run(m, {m[j]});
+            for (const Ev& e : ev)
+                if ((e.type == Ev::SCATTER && e.v == m[j]) ||
+                    (e.type == Ev::INSERT && e.idx == s && e.v == m[j] && e.enable))
+                    fail("apply() lets one vertex take two pattern vertices");
+        }
+        lab.push_back(LabelId(acc));
+        adj.push_back(srcs);
+        for (int j = 0; j < s; ++j)
+            if (srcs & (1u << j)) adj[j] |= 1u << s;
+        if (!inserts) {
+            if (s + 1 >= K_) fail("apply() scatters beyond the pattern size");
+            m.push_back(x);
+            continue;
+        }
+        // the pattern ends here: every position inserted with its vertex
+        const int k = s + 1;
+        std::vector<bool> seen(size_t(k), false);
+        for (const Ev& e : acc_ev) {
+            if (e.type != Ev::INSERT) continue;
+            if (e.idx < 0 || e.idx >= k) fail("apply() inserts at a pattern vertex it did not match");
+            const VertexId want = e.idx < s ? m[e.idx] : x;
+            if (e.v != want || !e.enable) fail("apply() inserts a vertex other than the matched one");
+            if (out->domain && out->domain != e.domain) fail("apply() inserts into two pattern domains");
+            out->domain = const_cast<void*>(e.domain);
+            seen[size_t(e.idx)] = true;
+        }
+        for (int i = 0; i < k; ++i)
+            if (!seen[size_t(i)]) fail("apply() leaves a pattern vertex out of its domain");
+        out->size = k;
+        for (int i = 0; i < k; ++i) {
+            out->labels[i] = lab[size_t(i)];
+            out->adj[i] = adj[size_t(i)];
+        }
+        return;
+    }
+}
+
+OpSink* g_prober = nullptr;
 
 }  // namespace
 
@@ -495,6 +762,23 @@ void probe_set_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet*
         std::exit(-1);
     }
     g_prober->on_op(op, lhs, rhs, result);
+}
+
+int extract_labeled_pattern(Khuzdul::ApplyFunction fn, int max_pattern_size, LabelId num_labels, bool root_known,
+                            LabelId root_label, LabeledPlan* out, std::string* error) {
+    LabelProber p(fn, max_pattern_size, num_labels);
+    g_prober = &p;
+    g_hooks = &p;
+    int rc = 0;
+    try {
+        p.extract(root_known, root_label, out);
+    } catch (const std::exception& ex) {
+        if (error) *error = ex.what();
+        rc = -1;
+    }
+    g_hooks = nullptr;
+    g_prober = nullptr;
+    return rc;
 }
 
 int extract_plan(Khuzdul::ApplyFunction fn, int max_pattern_size, const std::vector<void*>& aggregators,

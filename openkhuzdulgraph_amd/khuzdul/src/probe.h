@@ -16,6 +16,25 @@ namespace kzk {
 int extract_plan(Khuzdul::ApplyFunction fn, int max_pattern_size, const std::vector<void*>& aggregators,
                  kz_plan_desc* plan, std::string* error);
 
+// A labelled pattern recovered from a labelled app's apply(): `size`
+// vertices, adj[i] the bitmask of i's neighbours, labels[i] its label, and
+// the DistributedPatternDomain apply() inserts into.  `empty`: no embedding
+// can exist (a pattern vertex accepts no label, or the root label is not
+// the pattern's).
+struct LabeledPlan {
+    int size = 0;
+    uint32_t adj[MAX_PATTERN_SIZE] = {};
+    LabelId labels[MAX_PATTERN_SIZE] = {};
+    void* domain = nullptr;
+    bool empty = false;
+};
+
+// 0 on success; -1 with *error set when apply() is not a labelled pattern
+// walk the probe can recover.  root_known: the run restricts the roots to
+// root_label.
+int extract_labeled_pattern(Khuzdul::ApplyFunction fn, int max_pattern_size, LabelId num_labels, bool root_known,
+                            LabelId root_label, LabeledPlan* out, std::string* error);
+
 // Called by the out-of-line VertexSet operations.
 void probe_set_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result);
 
