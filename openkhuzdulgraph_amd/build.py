@@ -85,6 +85,7 @@ def build(verbose: bool = True) -> Path:
     # compiled unchanged against it where the reference tree is present
     khz = PKG / "khuzdul"
     subprocess.run(["make", "-s", "-B", "-C", str(khz)], check=True)
+    subprocess.run(["make", "-s", "-B", "-j4", "-C", str(khz), "examples"], check=True)
     if Path("/root/reference/distributed_applications").is_dir():
         subprocess.run(["make", "-s", "-j8", "-C", str(khz), "apps"], check=True)
     if verbose:

@@ -23,6 +23,12 @@ struct ProbeHooks {
         (void)v;
         (void)enable;
     }
+    // VertexSet::get_num_vertices: a read when the list is one the probe
+    // placed (a count added as a set's size), ignored otherwise
+    virtual void on_size(const VertexId* list, VertexId n) {
+        (void)list;
+        (void)n;
+    }
     virtual ~ProbeHooks() {}
 };
 

@@ -3,11 +3,15 @@
 //
 // intersect_with / subtracted_by keep the reference's contract
 // (src/vertex_set.cc:20-86: sorted A & B, sorted A \ B, written to the
-// result's list, length stored in the result).  On this engine they are
-// executed by the GPU kernels of libkz_hip.so as part of a plan; the
-// host-side symbols exist so that an unchanged apply() links, and are
-// evaluated only while DistributedApplication::run() probes apply() into a
-// plan.  Calling them from host code outside apply() is reported as an error.
+// result's list, length stored in the result), and so do their bounded
+// forms (src/vertex_set.cc:113-184: only ids below upper_bound, which must
+// be a matched vertex here) and intersect_and_count (:88-111, |A & B|).
+// On this engine they are executed by the GPU kernels of libkz_hip.so as
+// part of a plan; the host-side symbols exist so that an unchanged apply()
+// links, and are evaluated only while DistributedApplication::run() probes
+// apply() into a plan.  Calling them from host code outside apply() is
+// reported as an error.  get_num_vertices() reports a read of the list, so
+// a count added as a set's size is attributed to that set.
 #ifndef VERTEX_SET_H
 #define VERTEX_SET_H
 
@@ -32,7 +36,10 @@ struct VertexSet {
         if (__builtin_expect(kzk::g_hooks != nullptr, 0)) kzk::g_hooks->on_read(vertices_list_, num_vertices_);
         return vertices_list_;
     }
-    inline VertexId get_num_vertices() const { return num_vertices_; }
+    inline VertexId get_num_vertices() const {
+        if (__builtin_expect(kzk::g_hooks != nullptr, 0)) kzk::g_hooks->on_size(vertices_list_, num_vertices_);
+        return num_vertices_;
+    }
     inline VertexId get_vertex(VertexId idx) {
         if (__builtin_expect(kzk::g_hooks != nullptr, 0)) kzk::g_hooks->on_read(vertices_list_, num_vertices_);
         return vertices_list_[idx];

@@ -92,17 +92,12 @@ void VertexSet::intersect_with(VertexSet* op, VertexSet* result) {
     kzk::probe_set_op(KZ_OP_INTERSECT, this, op, result);
 }
 void VertexSet::subtracted_by(VertexSet* op, VertexSet* result) { kzk::probe_set_op(KZ_OP_SUBTRACT, this, op, result); }
-VertexId VertexSet::intersect_and_count(VertexSet*) {
-    std::fprintf(stderr, "VertexSet::intersect_and_count is not part of the GPU plan format\n");
-    std::exit(-1);
+VertexId VertexSet::intersect_and_count(VertexSet* op) { return kzk::probe_count_op(this, op); }
+void VertexSet::bounded_intersect_with(VertexSet* op, VertexSet* result, VertexId upper_bound) {
+    kzk::probe_set_op(KZ_OP_INTERSECT, this, op, result, upper_bound);
 }
-void VertexSet::bounded_intersect_with(VertexSet*, VertexSet*, VertexId) {
-    std::fprintf(stderr, "VertexSet::bounded_intersect_with is not part of the GPU plan format\n");
-    std::exit(-1);
-}
-void VertexSet::bounded_subtracted_by(VertexSet*, VertexSet*, VertexId) {
-    std::fprintf(stderr, "VertexSet::bounded_subtracted_by is not part of the GPU plan format\n");
-    std::exit(-1);
+void VertexSet::bounded_subtracted_by(VertexSet* op, VertexSet* result, VertexId upper_bound) {
+    kzk::probe_set_op(KZ_OP_SUBTRACT, this, op, result, upper_bound);
 }
 
 // ------------------------------------------------------------------ metrics --

@@ -30,6 +30,19 @@
 // length word sits just before the list, as create_cached_obj lays it out).
 // Types reached by a scatter of some class are explored depth-first.
 //
+// Bounded operations (bounded_intersect_with / bounded_subtracted_by,
+// src/vertex_set.cc:113-184: the result keeps only ids below upper_bound)
+// become plain operations whose bound lives in the predicates: the bound
+// must be a matched vertex m_j, every list derived from a bounded one
+// (intersect: either operand; subtract: the minuend) inherits the lowest
+// bound, and the probe writes X filtered by that bound into the result, so
+// a candidate loop over it learns "v < m_j" as missing predicate bits.  A
+// bounded subtrahend has no such form and is refused.
+// intersect_and_count (src/vertex_set.cc:88-111) is an intersect into a
+// slot of its own followed by a count loop over that slot: the probe logs
+// the operation, a read of the slot and returns |X| (filtered by the
+// operands' bounds), which apply() passes to Aggregator::add.
+//
 // Everything runs on the host, single-threaded, once per run(); it costs a
 // few thousand apply() calls on lists of at most a few ids.
 
@@ -56,6 +69,7 @@ namespace {
 
 constexpr VertexId kBase = 1u << 24;
 constexpr VertexId kGap = 1u << 16;
+constexpr VertexId kNoBound = ~VertexId(0);
 
 struct Prov {
     int kind = KZ_SET_NONE;  // KZ_SET_NBR / KZ_SET_SLOT
@@ -125,8 +139,11 @@ uint32_t code_of(const std::vector<int>& ranks) {
 }
 
 // Receiver of the VertexSet operations apply() issues while probed.
+// upper_bound: kNoBound, or the bound of a bounded_* operation.
 struct OpSink {
-    virtual void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) = 0;
+    virtual void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result,
+                       VertexId upper_bound) = 0;
+    virtual VertexId on_count(const VertexSet* lhs, const VertexSet* rhs) = 0;
     virtual ~OpSink() {}
 };
 
@@ -145,6 +162,9 @@ class Prober final : public ProbeHooks, public Khuzdul::AbstractEmbeddingExplora
         log_.push_back(e);
     }
     void on_trim(const VertexId* list, VertexId n) override { on_read(list, n); }
+    void on_size(const VertexId* list, VertexId n) override {
+        if (prov_.count(list)) on_read(list, n);
+    }
     void on_add(const void* agg, uint64_t delta) override {
         Event e;
         e.type = Event::ADD;
@@ -196,33 +216,39 @@ class Prober final : public ProbeHooks, public Khuzdul::AbstractEmbeddingExplora
 
     // VertexSet operations while probing (called from the out-of-line
     // VertexSet methods in khuzdul_api.cpp).
-    void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) override {
-        Event e;
-        e.type = Event::OP;
-        e.op = op;
-        e.lhs = prov_of(lhs->vertices_list_);
-        e.rhs = prov_of(rhs->vertices_list_);
-        const int idx = ops_this_call_++;
-        Prov dst;
-        dst.kind = KZ_SET_SLOT;
-        dst.a = cur_level_;
-        dst.b = idx;
-        log_.push_back(e);
-        // write the configured content into the result buffer
+    void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result,
+               VertexId upper_bound) override {
+        if (upper_bound != kNoBound && std::find(m_cur_.begin(), m_cur_.end(), upper_bound) == m_cur_.end())
+            fail("a bounded set operation's upper bound is not a matched vertex");
+        const VertexId b = std::min(upper_bound, result_bound(op, lhs, rhs));
+        log_op(op, lhs, rhs);
+        // write the configured content, cut at the bound, into the result buffer
+        const std::vector<VertexId> x = below(content_, b);
         const Region* r = region_of(result->vertices_list_);
         if (!r) fail("a set operation writes outside the scratchpad and cached objects");
         const size_t room = r->cap - size_t(result->vertices_list_ - r->start);
-        if (content_.size() > room) fail("probe content does not fit a result buffer");
-        std::copy(content_.begin(), content_.end(), result->vertices_list_);
-        result->num_vertices_ = VertexId(content_.size());
-        prov_[result->vertices_list_] = dst;
-        if (r->cached) cached_results_.push_back(CachedResult{result->vertices_list_, VertexId(content_.size())});
+        if (x.size() > room) fail("probe content does not fit a result buffer");
+        std::copy(x.begin(), x.end(), result->vertices_list_);
+        result->num_vertices_ = VertexId(x.size());
+        prov_[result->vertices_list_] = Prov{KZ_SET_SLOT, cur_level_, ops_this_call_ - 1};
+        bound_[result->vertices_list_] = b;
+        if (r->cached) cached_results_.push_back(CachedResult{result->vertices_list_, VertexId(x.size()), b});
+    }
+    VertexId on_count(const VertexSet* lhs, const VertexSet* rhs) override {
+        const VertexId b = result_bound(KZ_OP_INTERSECT, lhs, rhs);
+        log_op(KZ_OP_INTERSECT, lhs, rhs);
+        Event e;
+        e.type = Event::READ;
+        e.src = Prov{KZ_SET_SLOT, cur_level_, ops_this_call_ - 1};
+        log_.push_back(e);
+        return VertexId(below(content_, b).size());
     }
 
    private:
     struct CachedResult {
         VertexId* list;
         VertexId len;
+        VertexId bound;
     };
     struct OutKey {
         int kind;  // KZ_OUT_*
@@ -245,6 +271,32 @@ class Prober final : public ProbeHooks, public Khuzdul::AbstractEmbeddingExplora
         if (it == prov_.end()) fail("apply() used a vertex list the probe cannot attribute");
         return it->second;
     }
+    VertexId bound_of(const VertexId* p) const {
+        auto it = bound_.find(p);
+        return it == bound_.end() ? kNoBound : it->second;
+    }
+    // Bound a result inherits from its operands.
+    VertexId result_bound(int op, const VertexSet* lhs, const VertexSet* rhs) {
+        const VertexId rb = bound_of(rhs->vertices_list_);
+        if (op == KZ_OP_SUBTRACT && rb != kNoBound)
+            fail("apply() subtracts a bounded set; the plan format bounds results, not subtrahends");
+        return std::min(bound_of(lhs->vertices_list_), op == KZ_OP_INTERSECT ? rb : kNoBound);
+    }
+    static std::vector<VertexId> below(const std::vector<VertexId>& x, VertexId b) {
+        std::vector<VertexId> y;
+        for (VertexId v : x)
+            if (v < b) y.push_back(v);
+        return y;
+    }
+    void log_op(int op, const VertexSet* lhs, const VertexSet* rhs) {
+        Event e;
+        e.type = Event::OP;
+        e.op = op;
+        e.lhs = prov_of(lhs->vertices_list_);
+        e.rhs = prov_of(rhs->vertices_list_);
+        ++ops_this_call_;
+        log_.push_back(e);
+    }
     const Region* region_of(const VertexId* p) const {
         for (auto it = regions_.rbegin(); it != regions_.rend(); ++it)
             if (p >= it->start && p < it->start + it->cap) return &*it;
@@ -260,6 +312,8 @@ class Prober final : public ProbeHooks, public Khuzdul::AbstractEmbeddingExplora
     const std::vector<void*>& aggs_;
 
     std::map<const VertexId*, Prov> prov_;
+    std::map<const VertexId*, VertexId> bound_;  // results of bounded operations (and their descendants)
+    std::vector<VertexId> m_cur_;                // matched vertices of the apply() call running
     std::vector<Region> regions_;
     std::vector<std::unique_ptr<VertexId[]>> scratch_;
     std::vector<std::vector<VertexId>> nbr_bufs_;
@@ -283,6 +337,7 @@ void Prober::apply_at(int level, const std::vector<VertexId>& m, const std::vect
     std::memset(&e, 0, sizeof(e));
     e.size = level;
     content_ = content;
+    m_cur_.assign(m.begin(), m.begin() + level);
     for (int j = 0; j < level; ++j) {
         nbr_bufs_.emplace_back(content.empty() ? std::vector<VertexId>(1, 0) : content);
         VertexId* p = nbr_bufs_.back().data();
@@ -307,6 +362,7 @@ void Prober::apply_at(int level, const std::vector<VertexId>& m, const std::vect
 Prober::RunLog Prober::run_chain(const std::vector<VertexId>& m, const std::vector<VertexId>& X) {
     const int s = int(m.size());
     prov_.clear();
+    bound_.clear();
     regions_.clear();
     scratch_.clear();
     nbr_bufs_.clear();
@@ -329,11 +385,12 @@ Prober::RunLog Prober::run_chain(const std::vector<VertexId>& m, const std::vect
         cached[level].assign(std::max<size_t>(1 << 12, child_cached_bytes_ / 4 + 64), 0);
         regions_.push_back(Region{cached[level].data(), cached[level].size(), true, level + 1});
     }
-    // overwrite the ancestors' cached prefixes with X
+    // overwrite the ancestors' cached prefixes with X (cut at their bounds)
     for (const CachedResult& c : cached_results_) {
         if (c.list[-1] != c.len) fail("cached prefix without its length word before the list");
-        std::copy(X.begin(), X.end(), c.list);
-        c.list[-1] = VertexId(X.size());
+        const std::vector<VertexId> x = below(X, c.bound);
+        std::copy(x.begin(), x.end(), c.list);
+        c.list[-1] = VertexId(x.size());
     }
     log_.clear();
     follow_level_ = -1;
@@ -548,8 +605,14 @@ class LabelProber final : public ProbeHooks, public Khuzdul::AbstractEmbeddingEx
         if (it == label_.end()) fail("apply() reads the label of a vertex the probe did not place");
         return it->second;
     }
-    void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) override {
+    VertexId on_count(const VertexSet*, const VertexSet*) override {
+        fail("a labelled apply() counts an intersection (its pattern inserts each matched vertex)");
+        return 0;
+    }
+    void on_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result,
+               VertexId upper_bound) override {
         if (op != KZ_OP_INTERSECT) fail("a labelled apply() subtracts sets (induced patterns are not supported)");
+        if (upper_bound != kNoBound) fail("a labelled apply() bounds a set operation");
         Ev e;
         e.type = Ev::OP;
         e.lhs = prov_of(lhs->vertices_list_);
@@ -754,14 +817,27 @@ OpSink* g_prober = nullptr;
 
 }  // namespace
 
-void probe_set_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) {
+static void require_prober() {
     if (!g_prober) {
         std::fprintf(stderr,
                      "VertexSet set operations run on the GPU inside DistributedApplication::run(); "
                      "they cannot be called from host code outside apply().\n");
         std::exit(-1);
     }
-    g_prober->on_op(op, lhs, rhs, result);
+}
+
+void probe_set_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result, VertexId upper_bound) {
+    require_prober();
+    g_prober->on_op(op, lhs, rhs, result, upper_bound);
+}
+
+void probe_set_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result) {
+    probe_set_op(op, lhs, rhs, result, kNoBound);
+}
+
+VertexId probe_count_op(const VertexSet* lhs, const VertexSet* rhs) {
+    require_prober();
+    return g_prober->on_count(lhs, rhs);
 }
 
 int extract_labeled_pattern(Khuzdul::ApplyFunction fn, int max_pattern_size, LabelId num_labels, bool root_known,

@@ -35,8 +35,12 @@ struct LabeledPlan {
 int extract_labeled_pattern(Khuzdul::ApplyFunction fn, int max_pattern_size, LabelId num_labels, bool root_known,
                             LabelId root_label, LabeledPlan* out, std::string* error);
 
-// Called by the out-of-line VertexSet operations.
+// Called by the out-of-line VertexSet operations: intersect_with /
+// subtracted_by, their bounded forms (result ids below upper_bound) and
+// intersect_and_count (returns the count the probe's content gives).
 void probe_set_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result);
+void probe_set_op(int op, const VertexSet* lhs, const VertexSet* rhs, VertexSet* result, VertexId upper_bound);
+VertexId probe_count_op(const VertexSet* lhs, const VertexSet* rhs);
 
 }  // namespace kzk
 

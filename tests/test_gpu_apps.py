@@ -76,3 +76,29 @@ def test_plan_executor_matches_oracle_on_probed_plans(kzdev, tmp_path, monkeypat
                 want = oracle.run_plan(p, g.rowptr, g.col)
                 assert dg.count_plan(p) == want, (app, str(g))
             dg.release()
+
+
+EX_DIR = ROOT / "openkhuzdulgraph_amd" / "khuzdul" / "_examples"
+
+
+@pytest.mark.skipif(not EX_DIR.is_dir(), reason="example apps were not built")
+@pytest.mark.parametrize("method", ["auto", "plain"])
+def test_bounded_set_op_examples_print_exact_counts(method):
+    """intersect_and_count, bounded_intersect_with and bounded_subtracted_by
+    inside apply() (src/vertex_set.cc:88-184), through this package's
+    example apps: printed counts equal the reference binaries' triangle and
+    4-clique counts and the open-wedge closed form on the golden fixtures."""
+    from test_khuzdul_bounded_ops import FIXTURES, expected
+
+    for name in FIXTURES:
+        want = expected(name)
+        for app in sorted(want):
+            print(f"[case] app={app} graph={name} method={method}", flush=True)
+            env = dict(os.environ)
+            if method == "plain":
+                env["KZ_METHOD"] = "plain"
+            out = subprocess.run([str(EX_DIR / app), str(GOLDEN / f"{name}.dgraph")], env=env,
+                                 capture_output=True, text=True, timeout=120)
+            assert out.returncode == 0, out.stderr[-2000:]
+            got = {int(m.group(1)): int(m.group(2)) for m in PAT.finditer(out.stdout)}
+            assert [got[i] for i in sorted(got)] == want[app][0], (app, name, out.stdout[-1500:])
