@@ -41,6 +41,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -73,7 +74,7 @@ constexpr uint32_t kWaveTaskChunk = 32;
 // while the current one streams and counts.  Register budget: 6 waves per
 // SIMD (at most 80 VGPRs; the kernel is latency-bound, and 5 waves with 96
 // VGPRs measured 32.84 against 32.70 ms on RMAT-22 4-cliques, 7 32.91).
-template <int K>
+template <int K, bool QU = true>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void clique_wave_kernel(const QueryRef* __restrict__ queries,
                                                           const uint32_t* __restrict__ task_v,
                                                           const uint32_t* __restrict__ task_q, uint32_t ntasks,
@@ -88,6 +89,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
     __shared__ unsigned long long sRow[WPB][64];
     __shared__ unsigned long long sRb[WPB][64];
     __shared__ uint32_t sRp[WPB][64];
+    __shared__ uint2 sQ[QU ? WPB : 1][64];  // probe queues
     const uint32_t lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     uint32_t* L = sL[w];
@@ -146,11 +148,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
             un = task_v[tn];
         }
         wave_lds_sync();
-        stream_groups<kGroups, (K >= 4)>(col, rb, rp, d - 1, G, lane, 64, rs,
-                      [&](uint32_t r, uint32_t j, uint64_t) {
-                          if (K == 3) acc += 1;
-                          else atomicOr(&rows[r], 1ull << j);
-                      });
+        stream_groups<kGroups, (K >= 4), QU>(
+            col, rb, rp, d - 1, G, lane, 64, rs,
+            [&](uint32_t r, uint32_t j, uint64_t) {
+                if (K == 3) acc += 1;
+                else atomicOr(&rows[r], 1ull << j);
+            },
+            nullptr, QU ? sQ[QU ? w : 0] : nullptr);
         // next task's list bounds
         const uint64_t* rowptr_n = rowptr;
         const uint32_t* col_n = col;
@@ -281,10 +285,27 @@ __device__ __forceinline__ kz_v4i bits_to_bytes(uint32_t w, uint32_t off) {
     return r;
 }
 
+// Word kw of row i of a word-major bitmap is rows[word_base(kw) + i]: kw *
+// ws, or, packed (TRI), 32 kw (kw + 1).  A root's DAG rows are strictly
+// upper triangular (row i has bits j > i only), so word kw of rows i >=
+// 64 (kw + 1) - 1 is always zero: the packed form keeps rows i < 64 (kw +
+// 1) of word kw (all d rows of the last), tri_words(d) words in all.
+template <bool TRI>
+__device__ __forceinline__ uint32_t word_base(uint32_t kw, uint32_t ws) {
+    return TRI ? (kw * (kw + 1)) << 5 : kw * ws;
+}
+__host__ __device__ inline uint32_t tri_words(uint32_t d) {
+    const uint32_t W = (d + 63) >> 6;
+    return W ? 32 * (W - 1) * W + d : 0u;
+}
+
 // One 32 x 32 tile (I, J >= I) of A A^T masked by A: this lane's share of
 // sum A[i][j] * (A A^T)[i][j] over the tile.  rows is a word-major bitmap
-// (word kk of row i at rows[kk * ws + i]) in LDS or in HBM.  Call with
-// every lane of the wave.
+// (word kk of row i at rows[word_base<TRI>(kk, ws) + i]) in LDS or in HBM.
+// Every row a tile reads exists in the packed form: rows below 32 (J + 1)
+// from words kk >= J / 2, and rows clamped to d - 1 only in the last word.
+// Call with every lane of the wave.
+template <bool TRI = false>
 __device__ __forceinline__ uint32_t mfma4_tile(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t I,
                                                uint32_t J) {
     const uint32_t lane = threadIdx.x & 63;
@@ -302,13 +323,13 @@ __device__ __forceinline__ uint32_t mfma4_tile(const unsigned long long* rows, u
     for (int r = 0; r < 16; ++r) c[r] = 0;
     const uint32_t wend = (nb + 1) >> 1;
     uint32_t kw = J >> 1;
-    unsigned long long wi = rows[size_t(kw) * ws + ic], wj = rows[size_t(kw) * ws + jc];
+    unsigned long long wi = rows[word_base<TRI>(kw, ws) + ic], wj = rows[word_base<TRI>(kw, ws) + jc];
     if (J & 1u) {
         const uint32_t hi_i = uint32_t(wi >> 32), hi_j = uint32_t(wj >> 32);
         ++kw;
         if (kw < wend) {
-            wi = rows[size_t(kw) * ws + ic];
-            wj = rows[size_t(kw) * ws + jc];
+            wi = rows[word_base<TRI>(kw, ws) + ic];
+            wj = rows[word_base<TRI>(kw, ws) + jc];
         }
         c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(hi_i, hs), bits_to_bytes(hi_j, hs), c, 0, 0, 0);
     }
@@ -317,8 +338,8 @@ __device__ __forceinline__ uint32_t mfma4_tile(const unsigned long long* rows, u
         const uint32_t lo_j = uint32_t(wj), hi_j = uint32_t(wj >> 32);
         ++kw;
         if (kw < wend) {
-            wi = rows[size_t(kw) * ws + ic];
-            wj = rows[size_t(kw) * ws + jc];
+            wi = rows[word_base<TRI>(kw, ws) + ic];
+            wj = rows[word_base<TRI>(kw, ws) + jc];
         }
         c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(lo_i, hs), bits_to_bytes(lo_j, hs), c, 0, 0, 0);
         c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(hi_i, hs), bits_to_bytes(hi_j, hs), c, 0, 0, 0);
@@ -331,7 +352,7 @@ __device__ __forceinline__ uint32_t mfma4_tile(const unsigned long long* rows, u
     // are at most d < 2^24: 24-bit multiply-adds)
     uint32_t s = 0;
     if (j < d) {
-        const uint32_t* cw = reinterpret_cast<const uint32_t*>(rows + size_t(j >> 6) * ws) + ((j >> 5) & 1u);
+        const uint32_t* cw = reinterpret_cast<const uint32_t*>(rows + word_base<TRI>(j >> 6, ws)) + ((j >> 5) & 1u);
         const uint32_t b5 = j & 31u;
         const uint32_t ib = 32 * I + (lane >> 5) * 4;
         if (32 * I + 32 <= d) {  // every row of the tile exists (wave-uniform)
@@ -365,6 +386,7 @@ __device__ __forceinline__ void tile_coords(uint32_t t, uint32_t* I, uint32_t* J
 // before a barrier), the next tile's atomic issued ahead: tile t = J (J + 1)
 // / 2 + I, so the costly tiles (small J: the most 64-bit words) come first
 // and the waves finish together.
+template <bool TRI>
 __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows, uint32_t ws, uint32_t d,
                                                  uint32_t* ctr, unsigned long long& acc) {
     const uint32_t lane = threadIdx.x & 63;
@@ -378,18 +400,21 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
         if (lane == 0) tn = atomicAdd(ctr, 1u);
         uint32_t I, J;
         tile_coords(tile, &I, &J);
-        acc += mfma4_tile(rows, ws, d, I, J);
+        acc += mfma4_tile<TRI>(rows, ws, d, I, J);
         tile = wave_bcast(tn);
     }
 }
 
-// Sets bit j of row r in a word-major LDS bitmap (word kk of row r at
-// rows[kk * ws + r], ws < 2^24).  The word index is one v_mad_u32_u24
-// (written out: the compiler otherwise distributes the byte scaling over
-// the multiply and the add).
+// Sets bit j > r of row r in a word-major LDS bitmap (word kk of row r at
+// rows[word_base<TRI>(kk, ws) + r], ws < 2^24).  The word index is one
+// v_mad_u32_u24 (written out: the compiler otherwise distributes the byte
+// scaling over the multiply and the add).
+template <bool TRI = false>
 __device__ __forceinline__ void set_row_bit(unsigned long long* rows, uint32_t ws, uint32_t r, uint32_t j) {
     uint32_t w;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(w) : "v"(j >> 6), "s"(ws), "v"(r));
+    const uint32_t kw = j >> 6;
+    if (TRI) asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(w) : "v"(kw), "v"((kw << 5) + 32u), "v"(r));
+    else asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(w) : "v"(kw), "s"(ws), "v"(r));
     atomicOr(rows + w, 1ull << (j & 63));
 }
 
@@ -398,13 +423,15 @@ __device__ __forceinline__ void set_row_bit(unsigned long long* rows, uint32_t w
 // caller), streaming the neighbour lists of all rows as 16-byte groups.  rb holds
 // (r1 - r0) row descriptors and rp (r1 - r0 + 1) group prefixes, both LDS
 // scratch; tmp holds one u32 per wave and ctr one u32 (the stream's span
-// counter).  Block-wide; NG groups of 4 entries per lane and span.
-template <int K, int NG = kGroups>
+// counter).  Block-wide; NG groups of 4 entries per lane and span; QU:
+// queued probes, q the workgroup's queues (64 uint2 per wave); TRI: packed
+// rows (word_base).
+template <int K, int NG = kGroups, bool QU = false, bool TRI = false>
 __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
                                                  const uint32_t* L, uint32_t d, const RootSet& rs, uint32_t r0, uint32_t r1,
                                                  unsigned long long* rows_out, uint32_t ws, unsigned long long* rb,
                                                  uint32_t* rp, uint32_t* tmp, uint32_t* ctr,
-                                                 unsigned long long& acc) {
+                                                 unsigned long long& acc, uint2* q = nullptr) {
     const uint32_t nr = r1 - r0;
     if (threadIdx.x == 0) *ctr = 0;
     for (uint32_t r = threadIdx.x; r <= nr; r += blockDim.x) {
@@ -417,25 +444,28 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
     }
     __syncthreads();
     const uint32_t G = block_exclusive_scan(rp, nr + 1, tmp);
-    stream_groups<NG, (K >= 4)>(
+    stream_groups<NG, (K >= 4), QU>(
         col, rb, rp, nr, G, threadIdx.x, blockDim.x, rs,
         [&](uint32_t r, uint32_t j, uint64_t) {
             if (K == 3) acc += 1;
-            else set_row_bit(rows_out, ws, r, j);
+            else set_row_bit<TRI>(rows_out, ws, r, j);
         },
-        ctr);
+        ctr, QU ? q + (threadIdx.x >> 6) * 64 : nullptr);
     // skewed rows: each wave searches the candidates of its own rows
     search_skewed(col, rb, L, d, r0, nr, threadIdx.x >> 6, blockDim.x >> 6, [&](uint32_t r, uint32_t j, uint64_t) {
         if (K == 3) acc += 1;
-        else set_row_bit(rows_out, ws, r, j);
+        else set_row_bit<TRI>(rows_out, ws, r, j);
     });
     __syncthreads();
 }
 
 // ----------------------------------------------------------- block tasks --
-// One task per workgroup, rows in LDS.  Dynamic LDS: [rows Dmax*W u64 (k>=4)]
-// [row descriptors Dmax u64][hash keys H u32][L Dmax u32][group prefix
-// Dmax+1 u32][filter FW u32][hash vals H u16].  Register budget: WPE
+// One task per workgroup, rows in LDS.  Dynamic LDS: [rows Dmax*W u64 (k >= 4;
+// TRI, 4-cliques only: packed, tri_words(Dmax) u64)]
+// [row descriptors Dmax u64][probe queues 64 uint2 per wave (QU)][hash keys
+// H u32][L Dmax u32][group prefix Dmax+1 u32][filter FW u32][hash vals H u16].
+// QU (queued probes, where the LDS leaves room for them without costing a
+// workgroup per CU): the stream's filter passes are probed 64 at a time.  Register budget: WPE
 // waves per SIMD.  The default, 6 (at most 80 VGPRs), is the occupancy the
 // LDS of the middle buckets allows; there the latency-bound count phase
 // loses more to fewer waves than the stream gains from more registers.
@@ -445,7 +475,7 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
 // those whose LDS leaves 7 or more (the 64-128 bucket, latency-bound) use
 // WPE = 7 (72 VGPRs; 32.70 -> 32.43 ms, 8 waves no better), and so do
 // 3-cliques, which keep no bitmap (25.81 -> 25.46 ms).
-template <int K, int NG = kGroups, int WPE = 6>
+template <int K, int NG = kGroups, int WPE = 6, bool QU = false, bool TRI = false>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) void clique_block_kernel(const QueryRef* __restrict__ queries,
                                                            const uint32_t* __restrict__ task_v,
                                                            const uint32_t* __restrict__ task_q, uint32_t ntasks,
@@ -458,10 +488,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     unsigned long long* rows = nullptr;
     if (K >= 4) {
         rows = reinterpret_cast<unsigned long long*>(p);
-        p += size_t(Dmax) * W * 8;
+        p += size_t(TRI ? tri_words(Dmax) : Dmax * W) * 8;
     }
     unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
     p += size_t(Dmax) * 8;
+    uint2* qu = reinterpret_cast<uint2*>(p);
+    if (QU) p += size_t(blockDim.x) * 8;
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);  // 8-byte aligned (pair reads)
     p += size_t(H) * 4;
     uint32_t* L = reinterpret_cast<uint32_t*>(p);
@@ -499,12 +531,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) L[i] = ldg(col, ub + i);
         root_clear(rs, H, FW, threadIdx.x, blockDim.x);
         if (K >= 4)
-            for (uint32_t i = threadIdx.x; i < d * W; i += blockDim.x) rows[i] = 0;
+            for (uint32_t i = threadIdx.x, n = TRI ? tri_words(d) : d * W; i < n; i += blockDim.x) rows[i] = 0;
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
         __syncthreads();
-        build_rows_block<K, NG>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc);
-        if (K == 4) count_rows_mfma4(rows, d, d, &sTile, acc);
+        build_rows_block<K, NG, QU, TRI>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc, qu);
+        if (K == 4) count_rows_mfma4<TRI>(rows, d, d, &sTile, acc);
         else if (K == 5) count_rows<K>(rows, d, d, 0, 0, acc, &sTile);
         __syncthreads();
     }
@@ -721,9 +753,13 @@ static uint32_t filter_words(uint32_t Dmax) {
     return w;
 }
 
-static size_t block_lds_bytes(int K, uint32_t Dmax, uint32_t W, uint32_t H) {
-    return (K >= 4 ? size_t(Dmax) * W * 8 : 0) + size_t(Dmax) * 8 + size_t(Dmax) * 4 + size_t(Dmax + 1) * 4 +
-           size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
+// queue_threads: the workgroup size when it keeps probe queues, else 0;
+// tri: packed rows (4-cliques).
+static size_t block_lds_bytes(int K, uint32_t Dmax, uint32_t W, uint32_t H, uint32_t queue_threads = 0,
+                              bool tri = false) {
+    const size_t rows = K < 4 ? 0 : (tri && K == 4) ? size_t(tri_words(Dmax)) : size_t(Dmax) * W;
+    return rows * 8 + size_t(Dmax) * 8 + size_t(queue_threads) * 8 + size_t(Dmax) * 4 +
+           size_t(Dmax + 1) * 4 + size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
 }
 
 static size_t hub_lds_bytes(int K, uint32_t Dmax, uint32_t Wmax, uint32_t RPC, uint32_t H) {
@@ -744,6 +780,15 @@ static int set_lds_attrs() {
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 3, 4>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 2, 7>),
                          reinterpret_cast<const void*>(&clique_block_kernel<3, 2, 7>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<3, kGroups, 6, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4, kGroups, 6, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<5, kGroups, 6, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4, 3, 4, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4, 2, 7, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<3, 2, 7, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4, kGroups, 6, true, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4, 3, 4, true, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<4, 2, 7, true, true>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<3>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<4>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<5>)};
@@ -763,6 +808,44 @@ static void hash_size(uint32_t Dmax, uint32_t* H, uint32_t* hshift) {
     *hshift = 32 - bits;
 }
 
+// Arguments of clique_block_kernel after the launch configuration.
+struct BlockArgs {
+    const QueryRef* qrefs;
+    const uint32_t* tv;
+    const uint32_t* tq;
+    uint32_t nt;
+    uint32_t* cur;
+    unsigned long long* result;
+    uint32_t Dmax, W, H, hshift, FW, fshift;
+};
+
+template <int K, int NG, int WPE>
+static void launch_block(bool qu, bool tri, uint32_t grid, uint32_t block, size_t lds, hipStream_t s,
+                         const BlockArgs& a) {
+    if constexpr (K == 4) {
+        if (tri) {  // packed rows, with queues (where full rows and queues do not fit)
+            hipLaunchKernelGGL((clique_block_kernel<4, NG, WPE, true, true>), dim3(grid), dim3(block), lds, s, a.qrefs,
+                               a.tv, a.tq, a.nt, a.cur, a.result, a.Dmax, a.W, a.H, a.hshift, a.FW, a.fshift);
+            return;
+        }
+    }
+    if (qu)
+        hipLaunchKernelGGL((clique_block_kernel<K, NG, WPE, true>), dim3(grid), dim3(block), lds, s, a.qrefs, a.tv,
+                           a.tq, a.nt, a.cur, a.result, a.Dmax, a.W, a.H, a.hshift, a.FW, a.fshift);
+    else
+        hipLaunchKernelGGL((clique_block_kernel<K, NG, WPE, false>), dim3(grid), dim3(block), lds, s, a.qrefs, a.tv,
+                           a.tq, a.nt, a.cur, a.result, a.Dmax, a.W, a.H, a.hshift, a.FW, a.fshift);
+}
+
+// KZ_STREAM_QUEUE=0 turns the block buckets' probe queues off.
+static bool stream_queue_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("KZ_STREAM_QUEUE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 // One (k, bucket) launch (two kernels for hubs).
 template <int K>
 static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s) {
@@ -775,7 +858,12 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     const size_t lds_limit = R.lds_per_block - kStaticLdsReserve;
     if (gr.bucket == 0) {
         const uint32_t grid = std::min<uint32_t>((nt + 3) / 4, uint32_t(R.num_cus) * 8);
-        hipLaunchKernelGGL(clique_wave_kernel<K>, dim3(grid), dim3(256), 0, s, qrefs, tv, tq, nt, cur, x.d_result);
+        if (stream_queue_enabled())
+            hipLaunchKernelGGL((clique_wave_kernel<K, true>), dim3(grid), dim3(256), 0, s, qrefs, tv, tq, nt, cur,
+                               x.d_result);
+        else
+            hipLaunchKernelGGL((clique_wave_kernel<K, false>), dim3(grid), dim3(256), 0, s, qrefs, tv, tq, nt, cur,
+                               x.d_result);
         return KZ_OK;
     }
     if (gr.bucket < kNumBuckets - 1) {
@@ -783,36 +871,46 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         const uint32_t W = (Dmax + 63) / 64;
         uint32_t H, hshift;
         hash_size(Dmax, &H, &hshift);
-        const size_t lds = block_lds_bytes(K, Dmax, W, H);
         const uint32_t block = x.cfg.block[gr.bucket];
+        const size_t lds = block_lds_bytes(K, Dmax, W, H);
         if (lds > lds_limit)
             KZ_FAIL(KZ_ERR_UNSUPPORTED, "bucket %d needs %zu bytes of LDS (limit %zu)", gr.bucket, lds, lds_limit);
-        uint32_t per_cu = std::max<uint32_t>(1, uint32_t(R.lds_per_block / (lds + kStaticLdsReserve)));
-        per_cu = std::min<uint32_t>(per_cu, 2048 / block);
+        auto groups_per_cu = [&](size_t b) {
+            const uint32_t n = std::max<uint32_t>(1, uint32_t(R.lds_per_block / (b + kStaticLdsReserve)));
+            return std::min<uint32_t>(n, 2048 / block);
+        };
+        const uint32_t per_cu = groups_per_cu(lds);
+        // Probe queues where the LDS holds them at the same occupancy; for
+        // 4-cliques whose full rows leave no room for them (the 512-1024
+        // bucket), packed rows and queues.  Packed rows alone cost 0.4-2%
+        // per bucket (RMAT-24: 128-512 72.09 -> 72.37 ms, 64-128 8.50 ->
+        // 8.68), queues in the 512-1024 bucket gain 11% (66.09 -> 58.21).
+        const size_t ldsq = block_lds_bytes(K, Dmax, W, H, block);
+        const size_t ldst = block_lds_bytes(K, Dmax, W, H, block, true);
+        bool qu = stream_queue_enabled() && ldsq <= lds_limit && groups_per_cu(ldsq) == per_cu;
+        const bool tri = K == 4 && !qu && stream_queue_enabled() && ldst <= lds_limit && groups_per_cu(ldst) >= per_cu;
+        const size_t lds_run = tri ? ldst : qu ? ldsq : lds;
         const uint32_t grid = std::min<uint32_t>(nt, uint32_t(R.num_cus) * per_cu);
         uint32_t FW, fshift;
         filter_geometry(Dmax, &FW, &fshift);
+        const BlockArgs a{qrefs, tv, tq, nt, cur, x.d_result, Dmax, W, H, hshift, FW, fshift};
         if constexpr (K == 3) {  // no bitmap: the LDS leaves 7 waves per SIMD or more
             if (per_cu * block / 64 >= 28) {
-                hipLaunchKernelGGL((clique_block_kernel<3, 2, 7>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
-                                   cur, x.d_result, Dmax, W, H, hshift, FW, fshift);
+                launch_block<3, 2, 7>(qu, false, grid, block, lds_run, s, a);
                 return KZ_OK;
             }
         }
         if constexpr (K == 4) {
             if (per_cu * block / 64 >= 28) {  // LDS leaves 7 waves per SIMD or more
-                hipLaunchKernelGGL((clique_block_kernel<4, 2, 7>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
-                                   cur, x.d_result, Dmax, W, H, hshift, FW, fshift);
+                launch_block<4, 2, 7>(qu, tri, grid, block, lds_run, s, a);
                 return KZ_OK;
             }
             if (per_cu * block / 64 <= 16) {  // at most 4 waves per SIMD
-                hipLaunchKernelGGL((clique_block_kernel<4, 3, 4>), dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt,
-                                   cur, x.d_result, Dmax, W, H, hshift, FW, fshift);
+                launch_block<4, 3, 4>(qu, tri, grid, block, lds_run, s, a);
                 return KZ_OK;
             }
         }
-        hipLaunchKernelGGL(clique_block_kernel<K>, dim3(grid), dim3(block), lds, s, qrefs, tv, tq, nt, cur,
-                           x.d_result, Dmax, W, H, hshift, FW, fshift);
+        launch_block<K, kGroups, 6>(qu, tri, grid, block, lds_run, s, a);
         return KZ_OK;
     }
     // hubs: rows kernel, then (k >= 4) the count kernel on the same stream

@@ -223,10 +223,29 @@ __device__ __forceinline__ uint32_t wave_search(const uint32_t* p, uint32_t lo, 
 // span at a time with the next one's atomic issued ahead: the waves then
 // finish together instead of waiting at the caller's barrier for the one
 // whose share probed most.
-template <int NG, bool kIndex, class Hit>
+// Probes the n (wave-uniform, <= 64) queued candidates of a span, one per
+// lane: entry = (id, row << 16 | home pair).  hit(r, j, 0).
+template <bool kIndex, class Hit>
+__device__ __forceinline__ void drain_queue(const uint2* q, uint32_t n, const RootSet& rs, Hit& hit) {
+    wave_lds_sync();
+    const uint32_t lane = threadIdx.x & 63;
+    const bool act = lane < n;
+    const uint2 ent = q[lane];  // entries past n are stale and ignored
+    const uint32_t x = ent.x, r = ent.y >> 16, hp = act ? (ent.y & 0xFFFFu) : 0u;
+    const uint2 kp = reinterpret_cast<const uint2*>(rs.hk)[hp];
+    const uint32_t vp = kIndex ? reinterpret_cast<const uint32_t*>(rs.hv)[hp] : 0u;
+    const bool f0 = kp.x == x, f1 = kp.y == x;
+    if (act & (f0 | f1)) hit(r, f1 ? vp >> 16 : vp & 0xFFFFu, uint64_t(0));
+    if (act & !f0 & !f1 & (kp.x != kEmpty) & (kp.y != kEmpty)) {
+        const uint32_t j = hash_probe(rs.hk, rs.hv, rs.hmask, (2 * hp + 2) & rs.hmask, x);
+        if (j != kNotFound) hit(r, j, uint64_t(0));
+    }
+}
+
+template <int NG, bool kIndex, bool kQueue, class Hit>
 __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned long long* desc, const uint32_t* gp,
                                             uint32_t g0, uint32_t g1, uint32_t rlo, uint32_t rhi, const RootSet& rs,
-                                            Hit&& hit) {
+                                            uint2* q, Hit&& hit) {
     constexpr int NE = 4 * NG;
     const uint32_t lane = threadIdx.x & 63;
     // rows of this lane's groups (past g1: the group g1, masked below)
@@ -284,6 +303,30 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
             pass[e] = (c >= lo[e >> 2]) & (c < hi[e >> 2]) & (filter_has(fw[e], prod[e] >> foff) != 0u);
         }
     }
+    if constexpr (kQueue) {
+        // queued probes: the span's filter passes (about one id in ten) are
+        // packed into the wave's queue and probed 64 at a time, so the pair
+        // reads, compares and hit stores run once per candidate instead of
+        // once per id slot
+        uint32_t n = 0;  // wave-uniform
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const unsigned long long m = __ballot(pass[e]);
+            const uint32_t c = uint32_t(__popcll(m));
+            if (n + c > 64) {
+                drain_queue<kIndex>(q, n, rs, hit);
+                n = 0;
+            }
+            if (pass[e]) {
+                const uint32_t at_q =
+                    __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), n));
+                q[at_q] = make_uint2(y[e >> 2][e & 3], (rr[e >> 2] << 16) | (prod[e] >> (rs.hshift + 1)));
+            }
+            n += c;
+        }
+        if (n) drain_queue<kIndex>(q, n, rs, hit);
+        return;
+    }
     // home pairs (h & ~1, h | 1): keys with one 8-byte read, their indices
     // with one 4-byte read beside it (lanes without a candidate read pair 0:
     // a broadcast)
@@ -318,10 +361,12 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
     }
 }
 
-template <int NG = kGroups, bool kIndex = true, class Hit>
+// kQueue: q is this wave's queue of 64 uint2 in LDS for queued probes
+// (rows below 2^16, hash pairs below 2^16; hits then get p = 0).
+template <int NG = kGroups, bool kIndex = true, bool kQueue = false, class Hit>
 __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigned long long* desc, const uint32_t* gp,
                                               uint32_t nr, uint32_t G, uint32_t tid, uint32_t T, const RootSet& rs,
-                                              Hit&& hit, uint32_t* ctr = nullptr) {
+                                              Hit&& hit, uint32_t* ctr = nullptr, uint2* q = nullptr) {
     constexpr uint32_t SPAN = 64 * NG;
     const uint32_t lane = tid & 63;
     // the same in every lane: kept scalar
@@ -338,7 +383,7 @@ __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigne
             const uint32_t g0 = c * SPAN, g1 = (g0 + SPAN < G ? g0 + SPAN : G) - 1;
             const uint32_t rlo = wave_search(gp, 0, nr, g0);
             const uint32_t rhi = wave_search(gp, rlo, nr, g1) + 1;
-            stream_span<NG, kIndex>(col, desc, gp, g0, g1, rlo, rhi, rs, hit);
+            stream_span<NG, kIndex, kQueue>(col, desc, gp, g0, g1, rlo, rhi, rs, q, hit);
             c = wave_bcast(cn);
         }
         return;
@@ -353,7 +398,7 @@ __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigne
     for (uint32_t g0 = wbeg; g0 < wend; g0 += SPAN) {
         const uint32_t g1 = (g0 + SPAN < wend ? g0 + SPAN : wend) - 1;
         const uint32_t rhi = wave_search(gp, rlo, nr, g1) + 1;
-        stream_span<NG, kIndex>(col, desc, gp, g0, g1, rlo, rhi, rs, hit);
+        stream_span<NG, kIndex, kQueue>(col, desc, gp, g0, g1, rlo, rhi, rs, q, hit);
         rlo = rhi - 1;
     }
 }

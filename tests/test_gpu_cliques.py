@@ -143,6 +143,11 @@ def test_hub_and_bucket_paths_exact(kzdev, k):
     src = np.concatenate([rng.integers(0, 3000, 60000), np.zeros(2000, np.int64), np.ones(1500, np.int64)])
     dst = np.concatenate([rng.integers(0, 3000, 60000), np.arange(1, 2001), np.arange(2, 1502)])
     cases.append(("planted_hubs", _oriented_by_id(kz, kz.from_edges(src, dst, 3000))))
+    # two planted roots with forward degree in (512, 1024]: the last LDS
+    # bucket, whose 1024-thread workgroups keep packed rows and probe queues
+    src = np.concatenate([rng.integers(0, 3000, 60000), np.zeros(900, np.int64), np.ones(700, np.int64)])
+    dst = np.concatenate([rng.integers(0, 3000, 60000), np.arange(1, 901), np.arange(2, 702)])
+    cases.append(("planted_mid", _oriented_by_id(kz, kz.from_edges(src, dst, 3000))))
     for name, g in cases:
         print(f"[case] graph={name} k={k} max_degree={g.max_degree}", flush=True)
         want = oracle.run_app(APP[k], g.rowptr, g.col)[1][0]
