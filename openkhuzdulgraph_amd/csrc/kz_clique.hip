@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
     __shared__ unsigned long long sRow[WPB][64];
     __shared__ unsigned long long sRb[WPB][64];
     __shared__ uint32_t sRp[WPB][64];
-    __shared__ uint2 sQ[QU ? WPB : 1][64];  // probe queues
+    __shared__ __align__(8) uint32_t sQ[QU ? WPB : 1][128];  // probe queues
     const uint32_t lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     uint32_t* L = sL[w];
@@ -431,7 +431,7 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
                                                  const uint32_t* L, uint32_t d, const RootSet& rs, uint32_t r0, uint32_t r1,
                                                  unsigned long long* rows_out, uint32_t ws, unsigned long long* rb,
                                                  uint32_t* rp, uint32_t* tmp, uint32_t* ctr,
-                                                 unsigned long long& acc, uint2* q = nullptr) {
+                                                 unsigned long long& acc, uint32_t* q = nullptr) {
     const uint32_t nr = r1 - r0;
     if (threadIdx.x == 0) *ctr = 0;
     for (uint32_t r = threadIdx.x; r <= nr; r += blockDim.x) {
@@ -450,7 +450,7 @@ __device__ __forceinline__ void build_rows_block(const uint64_t* __restrict__ ro
             if (K == 3) acc += 1;
             else set_row_bit<TRI>(rows_out, ws, r, j);
         },
-        ctr, QU ? q + (threadIdx.x >> 6) * 64 : nullptr);
+        ctr, QU ? q + (threadIdx.x >> 6) * 128 : nullptr);
     // skewed rows: each wave searches the candidates of its own rows
     search_skewed(col, rb, L, d, r0, nr, threadIdx.x >> 6, blockDim.x >> 6, [&](uint32_t r, uint32_t j, uint64_t) {
         if (K == 3) acc += 1;
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     }
     unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
     p += size_t(Dmax) * 8;
-    uint2* qu = reinterpret_cast<uint2*>(p);
+    uint32_t* qu = reinterpret_cast<uint32_t*>(p);
     if (QU) p += size_t(blockDim.x) * 8;
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);  // 8-byte aligned (pair reads)
     p += size_t(H) * 4;

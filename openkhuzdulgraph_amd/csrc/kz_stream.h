@@ -124,24 +124,24 @@ struct RootSet {
     uint32_t fshift;
 };
 
-// Filter bit of x: the 32 - fshift bits just below the hash slot's in the
-// same Fibonacci product (hash_slot), so one 32-bit multiply serves both
-// (a filter wider than the bits below the slot uses those bits only).  The
-// bit starts at bit filter_offset of the product.
+// Filter of a root set: 2^(32 - fshift) bits in 32-bit words (fewer where
+// the bits below the hash slot run out).  An id's word is the bits of its
+// Fibonacci product (hash_slot) just below the slot's, its bit within the
+// word the product's low 5 bits (the id's low 5 bits under an odd
+// multiplier, a bijection), so one 32-bit multiply serves slot, word and
+// bit.  The word's bits start at filter_offset + 5 of the product.
 __device__ __forceinline__ uint32_t filter_offset(uint32_t hshift, uint32_t fshift) {
     const uint32_t fbits = 32 - fshift;
     return fbits < hshift ? hshift - fbits : 0u;
 }
-__device__ __forceinline__ uint32_t filter_bit_of(uint32_t prod, uint32_t hshift, uint32_t fshift) {
+__device__ __forceinline__ uint32_t filter_word_of(uint32_t prod, uint32_t hshift, uint32_t fshift) {
     const uint32_t off = filter_offset(hshift, fshift);
-    return __builtin_amdgcn_ubfe(prod, off, hshift - off);
+    return __builtin_amdgcn_ubfe(prod, off + 5, hshift - off - 5);
 }
-__device__ __forceinline__ uint32_t filter_bit(uint32_t x, uint32_t hshift, uint32_t fshift) {
-    return filter_bit_of(x * 2654435761u, hshift, fshift);
-}
-// Bit b of the filter, given its word.
-__device__ __forceinline__ uint32_t filter_has(uint32_t word, uint32_t b) {
-    return __builtin_amdgcn_ubfe(word, b & 31u, 1u);
+// The id's bit of its filter word: v_bfe_u32 reads its offset modulo 32,
+// so the product goes in as it is.
+__device__ __forceinline__ uint32_t filter_test(uint32_t word, uint32_t prod) {
+    return __builtin_amdgcn_ubfe(word, prod, 1u);
 }
 
 // Clears the hash (H slots) and filter (fw words) over threads tid..T.
@@ -163,8 +163,8 @@ __device__ __forceinline__ void root_insert(const RootSet& s, uint32_t x, uint16
         }
     }
     s.hv[h] = idx;
-    const uint32_t b = filter_bit(x, s.hshift, s.fshift);
-    atomicOr(&s.fb[b >> 5], 1u << (b & 31));
+    const uint32_t prod = x * 2654435761u;
+    atomicOr(&s.fb[filter_word_of(prod, s.hshift, s.fshift)], 1u << (prod & 31u));
 }
 
 // Filter geometry for lists of up to dmax ids: words and shift.
@@ -224,14 +224,15 @@ __device__ __forceinline__ uint32_t wave_search(const uint32_t* p, uint32_t lo, 
 // finish together instead of waiting at the caller's barrier for the one
 // whose share probed most.
 // Probes the n (wave-uniform, <= 64) queued candidates of a span, one per
-// lane: entry = (id, row << 16 | home pair).  hit(r, j, 0).
+// lane: entry i = (id, row) at q[2i], q[2i + 1].  hit(r, j, 0).
 template <bool kIndex, class Hit>
-__device__ __forceinline__ void drain_queue(const uint2* q, uint32_t n, const RootSet& rs, Hit& hit) {
+__device__ __forceinline__ void drain_queue(const uint32_t* q, uint32_t n, const RootSet& rs, Hit& hit) {
     wave_lds_sync();
     const uint32_t lane = threadIdx.x & 63;
     const bool act = lane < n;
-    const uint2 ent = q[lane];  // entries past n are stale and ignored
-    const uint32_t x = ent.x, r = ent.y >> 16, hp = act ? (ent.y & 0xFFFFu) : 0u;
+    const uint2 ent = reinterpret_cast<const uint2*>(q)[lane];  // entries past n are stale and ignored
+    const uint32_t x = ent.x, r = ent.y;
+    const uint32_t hp = act ? (x * 2654435761u) >> (rs.hshift + 1) : 0u;
     const uint2 kp = reinterpret_cast<const uint2*>(rs.hk)[hp];
     const uint32_t vp = kIndex ? reinterpret_cast<const uint32_t*>(rs.hv)[hp] : 0u;
     const bool f0 = kp.x == x, f1 = kp.y == x;
@@ -245,7 +246,7 @@ __device__ __forceinline__ void drain_queue(const uint2* q, uint32_t n, const Ro
 template <int NG, bool kIndex, bool kQueue, class Hit>
 __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned long long* desc, const uint32_t* gp,
                                             uint32_t g0, uint32_t g1, uint32_t rlo, uint32_t rhi, const RootSet& rs,
-                                            uint2* q, Hit&& hit) {
+                                            uint32_t* q, Hit&& hit) {
     constexpr int NE = 4 * NG;
     const uint32_t lane = threadIdx.x & 63;
     // rows of this lane's groups (past g1: the group g1, masked below)
@@ -284,48 +285,53 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
     }
     // filter (one multiply per id gives the filter bit and the hash slot;
     // the conditions are built without branches)
-    bool pass[NE];
-    uint32_t prod[NE];
-    {
-        const uint32_t foff = filter_offset(rs.hshift, rs.fshift);
-        const uint32_t fwid = rs.hshift - foff;  // filter bits, at least 6
-        uint32_t fw[NE];
+    const uint32_t foff = filter_offset(rs.hshift, rs.fshift);
+    const uint32_t fwid = rs.hshift - foff;  // filter bits, at least 6
+    uint32_t prod[NE], fw[NE];
 #pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            prod[e] = y[e >> 2][e & 3] * 2654435761u;
-            fw[e] = rs.fb[__builtin_amdgcn_ubfe(prod[e], foff + 5, fwid - 5)];
-        }
-#pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            const uint32_t c = uint32_t(e & 3);
-            // (no test against the root's largest id: its compare cost more
-            // than the probes it saved, 30.93 -> 30.71 ms on RMAT-22 4-cliques)
-            pass[e] = (c >= lo[e >> 2]) & (c < hi[e >> 2]) & (filter_has(fw[e], prod[e] >> foff) != 0u);
-        }
+    for (int e = 0; e < NE; ++e) {
+        prod[e] = y[e >> 2][e & 3] * 2654435761u;
+        fw[e] = rs.fb[__builtin_amdgcn_ubfe(prod[e], foff + 5, fwid - 5)];
     }
+    // (no test against the root's largest id: its compare cost more than the
+    // probes it saved, 30.93 -> 30.71 ms on RMAT-22 4-cliques)
     if constexpr (kQueue) {
         // queued probes: the span's filter passes (about one id in ten) are
-        // packed into the wave's queue and probed 64 at a time, so the pair
-        // reads, compares and hit stores run once per candidate instead of
-        // once per id slot
+        // packed into the wave's queue, (id, row) each, and probed 64 at a
+        // time, so the pair reads, compares and hit stores run once per
+        // candidate instead of once per id slot.  Each pass condition goes
+        // straight into its ballot (kept as a lane mask, not a flag).
+        // entries [lo, hi) of each group as a 4-bit mask (0 for dead groups)
+        uint32_t vm[NG];
+#pragma unroll
+        for (int k = 0; k < NG; ++k) vm[k] = (0xFu >> (4u - hi[k])) & (0xFu << lo[k]);
         uint32_t n = 0;  // wave-uniform
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            const unsigned long long m = __ballot(pass[e]);
+            const bool pe = (filter_test(fw[e], prod[e]) & __builtin_amdgcn_ubfe(vm[e >> 2], uint32_t(e & 3), 1u)) != 0u;
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(pe);  // the compare's mask as is
             const uint32_t c = uint32_t(__popcll(m));
             if (n + c > 64) {
                 drain_queue<kIndex>(q, n, rs, hit);
                 n = 0;
             }
-            if (pass[e]) {
-                const uint32_t at_q =
-                    __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), n));
-                q[at_q] = make_uint2(y[e >> 2][e & 3], (rr[e >> 2] << 16) | (prod[e] >> (rs.hshift + 1)));
+            if (pe) {
+                const uint32_t rank =
+                    __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+                uint32_t* slot = q + 2 * n + 2 * rank;
+                slot[0] = y[e >> 2][e & 3];
+                slot[1] = rr[e >> 2];
             }
             n += c;
         }
         if (n) drain_queue<kIndex>(q, n, rs, hit);
         return;
+    }
+    bool pass[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const uint32_t c = uint32_t(e & 3);
+        pass[e] = (c >= lo[e >> 2]) & (c < hi[e >> 2]) & (filter_test(fw[e], prod[e]) != 0u);
     }
     // home pairs (h & ~1, h | 1): keys with one 8-byte read, their indices
     // with one 4-byte read beside it (lanes without a candidate read pair 0:
@@ -361,12 +367,12 @@ __device__ __forceinline__ void stream_span(const uint32_t* col, const unsigned 
     }
 }
 
-// kQueue: q is this wave's queue of 64 uint2 in LDS for queued probes
-// (rows below 2^16, hash pairs below 2^16; hits then get p = 0).
+// kQueue: q is this wave's queue of 64 (id, row) pairs (128 u32) in LDS for
+// queued probes (hits then get p = 0).
 template <int NG = kGroups, bool kIndex = true, bool kQueue = false, class Hit>
 __device__ __forceinline__ void stream_groups(const uint32_t* col, const unsigned long long* desc, const uint32_t* gp,
                                               uint32_t nr, uint32_t G, uint32_t tid, uint32_t T, const RootSet& rs,
-                                              Hit&& hit, uint32_t* ctr = nullptr, uint2* q = nullptr) {
+                                              Hit&& hit, uint32_t* ctr = nullptr, uint32_t* q = nullptr) {
     constexpr uint32_t SPAN = 64 * NG;
     const uint32_t lane = tid & 63;
     // the same in every lane: kept scalar
