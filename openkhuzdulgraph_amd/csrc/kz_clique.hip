@@ -547,11 +547,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
 // Hub roots are split: item c of hub h builds rows [c*RPC, (c+1)*RPC) in an
 // LDS buffer and stores them to the hub's HBM slab (k >= 4), or counts the
 // matches directly (k = 3).  Dynamic LDS: [row buffer RPC*Wmax u64 (k>=4)]
-// [row descriptors RPC u64][hash keys H u32][L Dmax u32][group prefix
-// RPC+1 u32][filter FW u32][hash vals H u16].
+// [row descriptors RPC u64][probe queues 64 (id, row) per wave (QU)][hash
+// keys H u32][L Dmax u32][group prefix RPC+1 u32][filter FW u32][hash vals
+// H u16].
 constexpr uint32_t kHubItemRun = 8;
 
-template <int K>
+template <int K, bool QU = false>
 __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restrict__ queries,
                                                        const HubMeta* __restrict__ hubs, uint32_t nhubs,
                                                        const uint32_t* __restrict__ item_prefix, uint32_t nitems,
@@ -568,6 +569,8 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
     }
     unsigned long long* rb = reinterpret_cast<unsigned long long*>(p);
     p += size_t(RPC) * 8;
+    uint32_t* qu = reinterpret_cast<uint32_t*>(p);
+    if (QU) p += size_t(blockDim.x) * 8;
     uint32_t* hk = reinterpret_cast<uint32_t*>(p);  // 8-byte aligned (pair reads)
     p += size_t(H) * 4;
     uint32_t* L = reinterpret_cast<uint32_t*>(p);
@@ -622,7 +625,7 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
         if (K >= 4)
             for (uint32_t i = threadIdx.x; i < (r1 - r0) * hm.W; i += blockDim.x) rbuf[i] = 0;
         __syncthreads();
-        build_rows_block<K>(rowptr, col, L, d, rs, r0, r1, rbuf, r1 - r0, rb, rp, sTmp, &sSpan, acc);
+        build_rows_block<K, kGroups, QU>(rowptr, col, L, d, rs, r0, r1, rbuf, r1 - r0, rb, rp, sTmp, &sSpan, acc, qu);
         if (K >= 4) {
             // the slab is word-major over all d rows of the hub
             const uint32_t nr = r1 - r0;
@@ -762,9 +765,35 @@ static size_t block_lds_bytes(int K, uint32_t Dmax, uint32_t W, uint32_t H, uint
            size_t(Dmax + 1) * 4 + size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
 }
 
-static size_t hub_lds_bytes(int K, uint32_t Dmax, uint32_t Wmax, uint32_t RPC, uint32_t H) {
-    return (K >= 4 ? size_t(RPC) * Wmax * 8 : 0) + size_t(RPC) * 8 + size_t(RPC + 1) * 4 + size_t(Dmax) * 4 +
-           size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
+static size_t hub_lds_bytes(int K, uint32_t Dmax, uint32_t Wmax, uint32_t RPC, uint32_t H,
+                            uint32_t queue_threads = 0) {
+    return (K >= 4 ? size_t(RPC) * Wmax * 8 : 0) + size_t(RPC) * 8 + size_t(queue_threads) * 8 +
+           size_t(RPC + 1) * 4 + size_t(Dmax) * 4 + size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
+}
+
+// Launch shape of the hub rows kernel for row chunks of rpc rows: LDS bytes,
+// workgroup size and workgroups per CU (by LDS, and by its 80 VGPRs: 24
+// waves per CU).  1024-thread workgroups run one per CU; where the LDS
+// leaves room for three 512-thread ones, those fill the CU instead (RMAT-24
+// 4-cliques 211.2 -> 205.3 ms; where it does not, as for the forward
+// degrees of RMAT-26/ef32, 1024 threads stay faster: 6.42 against 7.95 s).
+constexpr uint32_t kHubWavesPerCU = 24;
+struct HubShape {
+    size_t lds;
+    uint32_t block, per_cu;
+};
+static HubShape hub_shape(int K, uint32_t dmax, uint32_t wmax, uint32_t rpc, uint32_t H, uint32_t cfg_block,
+                          bool queues) {
+    const size_t per_block = rt().lds_per_block;
+    auto fit = [&](size_t b) { return std::max<uint32_t>(1, uint32_t(per_block / (b + kStaticLdsReserve))); };
+    HubShape s{hub_lds_bytes(K, dmax, wmax, rpc, H, queues ? cfg_block : 0), cfg_block, 0};
+    s.per_cu = fit(s.lds);
+    if (cfg_block == 1024) {
+        const size_t l512 = hub_lds_bytes(K, dmax, wmax, rpc, H, queues ? 512 : 0);
+        if (fit(l512) >= 3) s = HubShape{l512, 512, fit(l512)};
+    }
+    s.per_cu = std::min<uint32_t>(s.per_cu, std::max<uint32_t>(1, kHubWavesPerCU / (s.block / 64)));
+    return s;
 }
 
 static int set_lds_attrs() {
@@ -791,7 +820,9 @@ static int set_lds_attrs() {
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 2, 7, true, true>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<3>),
                          reinterpret_cast<const void*>(&hub_rows_kernel<4>),
-                         reinterpret_cast<const void*>(&hub_rows_kernel<5>)};
+                         reinterpret_cast<const void*>(&hub_rows_kernel<5>),
+                         reinterpret_cast<const void*>(&hub_rows_kernel<4, true>),
+                         reinterpret_cast<const void*>(&hub_rows_kernel<5, true>)};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, maxb) != hipSuccess)
             (void)hipGetLastError();
@@ -915,25 +946,22 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     }
     // hubs: rows kernel, then (k >= 4) the count kernel on the same stream
     const CliqueJob::HubPlan& hp = x.hubplan[K];
-    uint32_t block = x.cfg.block[gr.bucket];
     uint32_t H, hshift;
     hash_size(hp.dmax, &H, &hshift);
     (void)hshift;
-    const size_t lds = hub_lds_bytes(K, hp.dmax, hp.wmax, hp.rpc, H);
-    if (lds > lds_limit) KZ_FAIL(KZ_ERR_UNSUPPORTED, "hub rows need %zu bytes of LDS (limit %zu)", lds, lds_limit);
-    uint32_t per_cu = std::max<uint32_t>(1, uint32_t(R.lds_per_block / (lds + kStaticLdsReserve)));
-    // 1024-thread workgroups of the hub rows kernel (80 VGPRs: 24 waves per
-    // CU) run one per CU; where the LDS leaves room for three 512-thread
-    // ones, those fill the CU instead (RMAT-24 4-cliques 211.2 -> 205.3 ms;
-    // where it does not, as for the forward degrees of RMAT-26/ef32, 1024
-    // threads stay faster: 6.42 against 7.95 s)
-    if (block == 1024 && per_cu >= 3) block = 512;
-    per_cu = std::min<uint32_t>(per_cu, 2048 / block);
-    const uint32_t grid = std::min<uint32_t>(hp.nitems, uint32_t(R.num_cus) * per_cu);
+    const HubShape sh = hub_shape(K, hp.dmax, hp.wmax, hp.rpc, H, x.cfg.block[gr.bucket], hp.queues);
+    if (sh.lds > lds_limit)
+        KZ_FAIL(KZ_ERR_UNSUPPORTED, "hub rows need %zu bytes of LDS (limit %zu)", sh.lds, lds_limit);
+    const uint32_t grid = std::min<uint32_t>(hp.nitems, uint32_t(R.num_cus) * sh.per_cu);
     const HubMeta* meta = static_cast<const HubMeta*>(x.hub_meta[K]);
-    hipLaunchKernelGGL(hub_rows_kernel<K>, dim3(grid), dim3(block), lds, s, qrefs, meta, hp.nhubs, x.hub_items[K],
-                       hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax, hp.wmax, hp.rpc, H,
-                       filter_words(hp.dmax));
+    if (hp.queues)
+        hipLaunchKernelGGL((hub_rows_kernel<K, true>), dim3(grid), dim3(sh.block), sh.lds, s, qrefs, meta, hp.nhubs,
+                           x.hub_items[K], hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax,
+                           hp.wmax, hp.rpc, H, filter_words(hp.dmax));
+    else
+        hipLaunchKernelGGL((hub_rows_kernel<K, false>), dim3(grid), dim3(sh.block), sh.lds, s, qrefs, meta, hp.nhubs,
+                           x.hub_items[K], hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax,
+                           hp.wmax, hp.rpc, H, filter_words(hp.dmax));
     if (K == 4) {
         const uint32_t cgrid = std::min<uint32_t>((hp.ntiles + 3) / 4, uint32_t(R.num_cus) * 8);
         hipLaunchKernelGGL(hub_mfma4_kernel, dim3(cgrid), dim3(256), 0, s, meta, hp.nhubs, x.hub_tiles[K], hp.ntiles,
@@ -962,6 +990,17 @@ static int prepare_hubs(CliqueJob& x, int k, const std::vector<uint32_t>& hv, co
     const size_t lds_limit = rt().lds_per_block - kStaticLdsReserve;
     hp.rpc = 64;
     while (hp.rpc > 1 && hub_lds_bytes(k, hp.dmax, hp.wmax, hp.rpc, H) > lds_limit) hp.rpc >>= 1;
+    // probe queues (k >= 4: the instances within 80 VGPRs) where the LDS
+    // holds them beside the same chunk with as many threads per CU.  Not by
+    // halving the chunk: rows of 32 instead of 64 with queues measured
+    // slower (RMAT-24 hub rows 17.53 -> 18.83 ms, RMAT-26/ef32 2.93 ->
+    // 3.12 s, there 16 instead of 32).
+    if (k >= 4 && stream_queue_enabled()) {
+        const uint32_t cfg_block = x.cfg.block[kNumBuckets - 1];
+        const HubShape s0 = hub_shape(k, hp.dmax, hp.wmax, hp.rpc, H, cfg_block, false);
+        const HubShape sq = hub_shape(k, hp.dmax, hp.wmax, hp.rpc, H, cfg_block, true);
+        hp.queues = sq.lds <= lds_limit && sq.per_cu * sq.block >= s0.per_cu * s0.block;
+    }
     std::vector<HubMeta> meta(hp.nhubs);
     std::vector<uint32_t> items(hp.nhubs), rows(hp.nhubs), tiles(hp.nhubs);
     uint64_t slab = 0, nt = 0;

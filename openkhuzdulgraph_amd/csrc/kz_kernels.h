@@ -268,6 +268,7 @@ struct CliqueJob {
     // workgroups; rows go to an HBM slab, then one wave counts each row
     struct HubPlan {
         uint32_t nhubs = 0, dmax = 0, wmax = 0, rpc = 64, nitems = 0, nrows = 0, ntiles = 0;
+        bool queues = false;  // rows kernel with probe queues
     };
     std::map<int, HubPlan> hubplan;
     std::map<int, void*> hub_meta;         // HubMeta[nhubs]
