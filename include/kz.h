@@ -391,7 +391,9 @@ typedef struct kz_query_result {
 int kz_server_start(const kz_server_cfg* cfg, kz_server* out);
 int kz_server_submit(kz_server s, kz_graph g, int kind, int k, uint64_t* ticket);
 /* KZ_OK: done, *out filled and the ticket retired; KZ_PENDING: not done
- * (wait == 0 only); wait != 0 blocks until the query finishes. */
+ * (wait == 0 only); wait != 0 blocks until the query finishes.  For a
+ * query that failed (out->status != KZ_OK), kz_last_error() in the calling
+ * thread then returns the failure's message. */
 int kz_server_poll(kz_server s, uint64_t ticket, int wait, kz_query_result* out);
 /* Finishes every submitted query, stops the worker, releases the prepared
  * executions the server holds and frees the server. */

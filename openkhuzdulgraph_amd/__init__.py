@@ -323,7 +323,10 @@ class Server:
             return None
         check(rc)
         self._graphs.pop(ticket, None)
-        return r.as_dict()
+        res = r.as_dict()
+        if res["status"]:
+            res["error"] = lib().kz_last_error().decode(errors="replace")
+        return res
 
     def stop(self) -> None:
         if self._h:

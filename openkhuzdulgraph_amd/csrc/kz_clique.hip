@@ -299,6 +299,38 @@ __host__ __device__ inline uint32_t tri_words(uint32_t d) {
     return W ? 32 * (W - 1) * W + d : 0u;
 }
 
+// This lane's share of sum_r A[i_r][j] * c[r] over its 16 rows i_r of tile
+// row block I (accumulator r of lane l holds row (r / 4) * 8 + (l / 32) * 4 +
+// r % 4) and its column j: A[i_r][j] read as the 32-bit half of each row's
+// word that holds bit j (entries of A A^T are at most d < 2^24: 24-bit
+// multiply-adds).
+template <bool TRI>
+__device__ __forceinline__ uint32_t mask_tile(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t I,
+                                              uint32_t j, const kz_v16i& c) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t s = 0;
+    if (j < d) {
+        const uint32_t* cw = reinterpret_cast<const uint32_t*>(rows + word_base<TRI>(j >> 6, ws)) + ((j >> 5) & 1u);
+        const uint32_t b5 = j & 31u;
+        const uint32_t ib = 32 * I + (lane >> 5) * 4;
+        if (32 * I + 32 <= d) {  // every row of the tile exists (wave-uniform)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
+                s = __umul24(__builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u), uint32_t(c[r])) + s;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
+                const uint32_t a = ir < d ? __builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u) : 0u;
+                s = __umul24(a, uint32_t(c[r])) + s;
+            }
+        }
+    }
+    return s;
+}
+
 // One 32 x 32 tile (I, J >= I) of A A^T masked by A: this lane's share of
 // sum A[i][j] * (A A^T)[i][j] over the tile.  rows is a word-major bitmap
 // (word kk of row i at rows[word_base<TRI>(kk, ws) + i]) in LDS or in HBM.
@@ -347,29 +379,71 @@ __device__ __forceinline__ uint32_t mfma4_tile(const unsigned long long* rows, u
     if (2 * kw < nb)
         c = __builtin_amdgcn_mfma_i32_32x32x32_i8(bits_to_bytes(uint32_t(wi), hs), bits_to_bytes(uint32_t(wj), hs), c,
                                                   0, 0, 0);
-    // mask: A[i_r][j] for this lane's 16 rows i_r and its column j, read as
-    // the 32-bit half of each row's word that holds bit j (entries of A A^T
-    // are at most d < 2^24: 24-bit multiply-adds)
-    uint32_t s = 0;
-    if (j < d) {
-        const uint32_t* cw = reinterpret_cast<const uint32_t*>(rows + word_base<TRI>(j >> 6, ws)) + ((j >> 5) & 1u);
-        const uint32_t b5 = j & 31u;
-        const uint32_t ib = 32 * I + (lane >> 5) * 4;
-        if (32 * I + 32 <= d) {  // every row of the tile exists (wave-uniform)
+    return mask_tile<TRI>(rows, ws, d, I, j, c);
+}
+
+// Four 32 x 32 tiles (2 bI + a, 2 bJ + b), a, b in {0, 1}, of A A^T masked
+// by A in one pass over the 64-bit words kw >= bJ (block (bI, bJ), bJ >=
+// bI, of the W x W word blocks).  Each word's operands, the rows of the two
+// I tiles and of the two J tiles, are expanded to bytes once and feed four
+// MFMAs per 32-column half: 12 VALU of expansion per MFMA instead of 24, and
+// half the row reads.  Tile 2 bJ + 1 also takes the low half of word bJ,
+// whose products are zero (row j has no bit l <= j).  Tiles below the
+// diagonal (2 bI + 1 > 2 bJ) or past the last row are computed on clamped
+// rows and left out; a clamped row, d - 1, is empty.  The accumulators take
+// 64 registers: for kernels with a budget of at least 128.
+template <bool TRI>
+__device__ __forceinline__ uint32_t mfma4_block(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t bI,
+                                                uint32_t bJ) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nb = (d + 31) >> 5, W = (d + 63) >> 6;
+    const uint32_t i0 = 64 * bI + (lane & 31), j0 = 64 * bJ + (lane & 31);
+    const uint32_t ic0 = i0 < d ? i0 : d - 1, ic1 = i0 + 32 < d ? i0 + 32 : d - 1;
+    const uint32_t jc0 = j0 < d ? j0 : d - 1, jc1 = j0 + 32 < d ? j0 + 32 : d - 1;
+    const uint32_t hs = 16 * (lane >> 5);
+    kz_v16i c00, c01, c10, c11;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
-                s = __umul24(__builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u), uint32_t(c[r])) + s;
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t ir = ib + (r >> 2) * 8 + (r & 3);
-                const uint32_t a = ir < d ? __builtin_amdgcn_ubfe(cw[2 * ir], b5, 1u) : 0u;
-                s = __umul24(a, uint32_t(c[r])) + s;
-            }
+    for (int r = 0; r < 16; ++r) c00[r] = c01[r] = c10[r] = c11[r] = 0;
+    uint32_t kw = bJ;
+    unsigned long long a0 = rows[word_base<TRI>(kw, ws) + ic0], a1 = rows[word_base<TRI>(kw, ws) + ic1];
+    unsigned long long b0 = rows[word_base<TRI>(kw, ws) + jc0], b1 = rows[word_base<TRI>(kw, ws) + jc1];
+    while (kw < W) {
+        const uint32_t kn = kw + 1;
+        unsigned long long na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
+        if (kn < W) {
+            const uint32_t base = word_base<TRI>(kn, ws);
+            na0 = rows[base + ic0];
+            na1 = rows[base + ic1];
+            nb0 = rows[base + jc0];
+            nb1 = rows[base + jc1];
         }
+        {
+            const kz_v4i A0 = bits_to_bytes(uint32_t(a0), hs), A1 = bits_to_bytes(uint32_t(a1), hs);
+            const kz_v4i B0 = bits_to_bytes(uint32_t(b0), hs), B1 = bits_to_bytes(uint32_t(b1), hs);
+            c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A0, B0, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A0, B1, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A1, B0, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A1, B1, c11, 0, 0, 0);
+        }
+        if (2 * kw + 1 < nb) {  // the word's high half holds columns below d
+            const kz_v4i A0 = bits_to_bytes(uint32_t(a0 >> 32), hs), A1 = bits_to_bytes(uint32_t(a1 >> 32), hs);
+            const kz_v4i B0 = bits_to_bytes(uint32_t(b0 >> 32), hs), B1 = bits_to_bytes(uint32_t(b1 >> 32), hs);
+            c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A0, B0, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A0, B1, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A1, B0, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A1, B1, c11, 0, 0, 0);
+        }
+        a0 = na0;
+        a1 = na1;
+        b0 = nb0;
+        b1 = nb1;
+        kw = kn;
     }
+    const uint32_t I0 = 2 * bI, J0 = 2 * bJ;
+    uint32_t s = mask_tile<TRI>(rows, ws, d, I0, j0, c00);
+    if (J0 + 1 < nb) s += mask_tile<TRI>(rows, ws, d, I0, j0 + 32, c01);
+    if (I0 + 1 < nb && I0 + 1 <= J0) s += mask_tile<TRI>(rows, ws, d, I0 + 1, j0, c10);
+    if (J0 + 1 < nb && I0 + 1 < nb) s += mask_tile<TRI>(rows, ws, d, I0 + 1, j0 + 32, c11);
     return s;
 }
 
@@ -386,11 +460,13 @@ __device__ __forceinline__ void tile_coords(uint32_t t, uint32_t* I, uint32_t* J
 // before a barrier), the next tile's atomic issued ahead: tile t = J (J + 1)
 // / 2 + I, so the costly tiles (small J: the most 64-bit words) come first
 // and the waves finish together.
-template <bool TRI>
+// BLK: 2 x 2 tile blocks (mfma4_block) instead of single tiles, for kernels
+// whose register budget holds their four accumulators.
+template <bool TRI, bool BLK = false>
 __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows, uint32_t ws, uint32_t d,
                                                  uint32_t* ctr, unsigned long long& acc) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t nb = (d + 31) >> 5;
+    const uint32_t nb = BLK ? (d + 63) >> 6 : (d + 31) >> 5;
     const uint32_t ntiles = nb * (nb + 1) / 2;
     uint32_t tile = 0;
     if (lane == 0) tile = atomicAdd(ctr, 1u);
@@ -400,7 +476,7 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
         if (lane == 0) tn = atomicAdd(ctr, 1u);
         uint32_t I, J;
         tile_coords(tile, &I, &J);
-        acc += mfma4_tile<TRI>(rows, ws, d, I, J);
+        acc += BLK ? mfma4_block<TRI>(rows, ws, d, I, J) : mfma4_tile<TRI>(rows, ws, d, I, J);
         tile = wave_bcast(tn);
     }
 }
@@ -536,7 +612,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
         __syncthreads();
         build_rows_block<K, NG, QU, TRI>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc, qu);
-        if (K == 4) count_rows_mfma4<TRI>(rows, d, d, &sTile, acc);
+        if (K == 4) count_rows_mfma4<TRI, false>(rows, d, d, &sTile, acc);
         else if (K == 5) count_rows<K>(rows, d, d, 0, 0, acc, &sTile);
         __syncthreads();
     }
@@ -665,11 +741,11 @@ __global__ __launch_bounds__(256) void hub_count_kernel(const HubMeta* __restric
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
 
-// 4-cliques of the hubs on MFMA: one wave per 32 x 32 tile of a hub's
-// A A^T, the tiles of every hub (tile_prefix: exclusive prefix over hubs of
-// nb (nb + 1) / 2, nb = ceil(d / 32)) taken from one cursor, read from the
-// hub's word-major slab rows in HBM.
-__global__ __launch_bounds__(256) void hub_mfma4_kernel(const HubMeta* __restrict__ hubs, uint32_t nhubs,
+// 4-cliques of the hubs on MFMA: one wave per 2 x 2 block of 32 x 32 tiles
+// of a hub's A A^T (mfma4_block), the blocks of every hub (tile_prefix:
+// exclusive prefix over hubs of W (W + 1) / 2, W = ceil(d / 64)) taken from
+// one cursor, read from the hub's word-major slab rows in HBM.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void hub_mfma4_kernel(const HubMeta* __restrict__ hubs, uint32_t nhubs,
                                                         const uint32_t* __restrict__ tile_prefix, uint32_t ntiles,
                                                         uint32_t* __restrict__ cursor,
                                                         unsigned long long* __restrict__ results,
@@ -688,7 +764,7 @@ __global__ __launch_bounds__(256) void hub_mfma4_kernel(const HubMeta* __restric
         }
         uint32_t I, J;
         tile_coords(t - tile_prefix[h], &I, &J);
-        acc += mfma4_tile(slab + hm.slab_off, hm.d, hm.d, I, J);
+        acc += mfma4_block<false>(slab + hm.slab_off, hm.d, hm.d, I, J);
     }
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
@@ -1012,7 +1088,7 @@ static int prepare_hubs(CliqueJob& x, int k, const std::vector<uint32_t>& hv, co
     for (uint32_t h = 0; h < hp.nhubs; ++h) {
         const uint32_t d = hd[h];
         const uint32_t W = (d + 63) / 64;
-        const uint64_t nb = (d + 31) / 32;
+        const uint64_t nb = (d + 63) / 64;  // 2 x 2 tile blocks: one per pair of 64-bit words
         meta[h] = HubMeta{hv[h], hq[h], d, W, slab};
         slab += uint64_t(d) * W;
         items[h] = ni;

@@ -48,6 +48,7 @@ struct SQuery {
     int kind = 0, k = 0;
     Clock::time_point t_submit, t_start;
     kz_query_result res{};
+    std::string err;  // kz_last_error() of the call that failed the query
     bool done = false;
 };
 
@@ -75,6 +76,7 @@ struct AuxJob {
     Prepared* p = nullptr;
     int rc = 0;
     double ms = 0;
+    std::string err;  // the helper thread's kz_last_error() when rc != 0
 };
 
 struct AuxWorker {
@@ -243,8 +245,11 @@ bool any_busy(const kz_server_s* s) {
     return false;
 }
 
-void finish(kz_server_s* s, SQuery* q, int status) {
+// err: the message of a failure on another thread (a helper's), else the
+// calling thread's own kz_last_error() is kept.
+void finish(kz_server_s* s, SQuery* q, int status, const std::string& err = std::string()) {
     q->res.status = status;
+    if (status) q->err = err.empty() ? std::string(get_error()) : err;
     const Clock::time_point t = Clock::now();
     q->res.total_ms = ms_between(q->t_submit, t);
     if (q->t_start == Clock::time_point{}) q->t_start = t;
@@ -270,6 +275,7 @@ void aux_run(AuxJob& j) {
         j.q->t_start = t0;
         j.rc = motif_census(d, j.q->res.counts, &st);
     }
+    if (j.rc) j.err = get_error();
     j.ms = ms_between(t0, Clock::now());
 }
 
@@ -368,7 +374,7 @@ void collect_aux(kz_server_s* s, std::vector<SQuery*>* gone) {
             if (!clique_prepared_on(s, g) && !find_prepared(s, g, kForwardCopy, 0) && d.fwd_ready &&
                 !clique_exec_on(d))
                 release_forward(d);
-            finish(s, j.q, j.rc);
+            finish(s, j.q, j.rc, j.err);
             gone->push_back(j.q);
             continue;
         }
@@ -392,7 +398,7 @@ void collect_aux(kz_server_s* s, std::vector<SQuery*>* gone) {
             p->busy--;
             touch(s, p);
         }
-        finish(s, j.q, j.rc);
+        finish(s, j.q, j.rc, j.err);
     }
 }
 
@@ -666,8 +672,10 @@ int kz_server_poll(kz_server s, uint64_t ticket, int wait, kz_query_result* out)
         s->cv_done.wait(lk, [&] { return q->done; });
     }
     *out = q->res;
+    const std::string err = q->err;
     s->live.erase(ticket);
     delete q;
+    if (out->status) set_error("%s", err.c_str());  // the failed query's message, for this thread
     return KZ_OK;
 }
 
