@@ -1244,6 +1244,8 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         KZ_HIP(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
     }
     KZ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    // the launches' errors are read after the capture: start from none
+    (void)hipGetLastError();
     KZ_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(reset_u32_kernel, dim3(1), dim3(256), 0, main, x.d_cursor, std::max(1, cursor));
     hipLaunchKernelGGL(reset_u64_kernel, dim3(1), dim3(256), 0, main, x.d_result,
@@ -1266,9 +1268,12 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         }
     hipGraph_t graph;
     KZ_HIP(hipStreamEndCapture(main, &graph));
-    if (e == KZ_OK && hipGetLastError() != hipSuccess) {
-        e = KZ_ERR_DEVICE;
-        kz::set_error("a bucket launch was rejected during capture");
+    if (e == KZ_OK) {
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess) {
+            e = KZ_ERR_DEVICE;
+            kz::set_error("a bucket launch was rejected during capture: %s", hipGetErrorString(le));
+        }
     }
     for (size_t i = 0; i < nbr; ++i) {
         hipStreamDestroy(br[i]);
