@@ -460,14 +460,79 @@ __device__ __forceinline__ void tile_coords(uint32_t t, uint32_t* I, uint32_t* J
 // before a barrier), the next tile's atomic issued ahead: tile t = J (J + 1)
 // / 2 + I, so the costly tiles (small J: the most 64-bit words) come first
 // and the waves finish together.
-// BLK: 2 x 2 tile blocks (mfma4_block) instead of single tiles, for kernels
-// whose register budget holds their four accumulators.
+// Two 32 x 32 tiles (I, 2 bJ) and (I, 2 bJ + 1), I <= 2 bJ + 1, of A A^T
+// masked by A in one pass over the words kw >= bJ: the row block I operand
+// is expanded once per half for two MFMAs (18 VALU of expansion per MFMA
+// instead of 24); two accumulators (32 registers).  Zero products and
+// clamped rows as in mfma4_block.
+template <bool TRI>
+__device__ __forceinline__ uint32_t mfma4_pair(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t I,
+                                               uint32_t bJ) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nb = (d + 31) >> 5, W = (d + 63) >> 6;
+    const uint32_t i = 32 * I + (lane & 31), j0 = 64 * bJ + (lane & 31);
+    const uint32_t ic = i < d ? i : d - 1;
+    const uint32_t jc0 = j0 < d ? j0 : d - 1, jc1 = j0 + 32 < d ? j0 + 32 : d - 1;
+    const uint32_t hs = 16 * (lane >> 5);
+    kz_v16i c0, c1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) c0[r] = c1[r] = 0;
+    uint32_t kw = bJ;
+    unsigned long long a = rows[word_base<TRI>(kw, ws) + ic];
+    unsigned long long b0 = rows[word_base<TRI>(kw, ws) + jc0], b1 = rows[word_base<TRI>(kw, ws) + jc1];
+    while (kw < W) {
+        const uint32_t kn = kw + 1;
+        unsigned long long na = 0, nb0 = 0, nb1 = 0;
+        if (kn < W) {
+            const uint32_t base = word_base<TRI>(kn, ws);
+            na = rows[base + ic];
+            nb0 = rows[base + jc0];
+            nb1 = rows[base + jc1];
+        }
+        {
+            const kz_v4i A = bits_to_bytes(uint32_t(a), hs);
+            c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, bits_to_bytes(uint32_t(b0), hs), c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, bits_to_bytes(uint32_t(b1), hs), c1, 0, 0, 0);
+        }
+        if (2 * kw + 1 < nb) {
+            const kz_v4i A = bits_to_bytes(uint32_t(a >> 32), hs);
+            c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, bits_to_bytes(uint32_t(b0 >> 32), hs), c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, bits_to_bytes(uint32_t(b1 >> 32), hs), c1, 0, 0, 0);
+        }
+        a = na;
+        b0 = nb0;
+        b1 = nb1;
+        kw = kn;
+    }
+    uint32_t s = 0;
+    if (I <= 2 * bJ) s = mask_tile<TRI>(rows, ws, d, I, j0, c0);
+    if (2 * bJ + 1 < nb) s += mask_tile<TRI>(rows, ws, d, I, j0 + 32, c1);
+    return s;
+}
+
+// (I, bJ) of pair t: pairs come by bJ, I = 0 .. 2 bJ + 1 (0 .. nb - 1 for
+// the last word), so pair t of bJ < W - 1 is t = bJ (bJ + 1) + I.
+__device__ __forceinline__ void pair_coords(uint32_t t, uint32_t W, uint32_t* I, uint32_t* bJ) {
+    if (t >= (W - 1) * W) {
+        *bJ = W - 1;
+        *I = t - (W - 1) * W;
+        return;
+    }
+    uint32_t b = uint32_t((sqrtf(4.0f * float(t) + 1.0f) - 1.0f) * 0.5f);
+    if (b * (b + 1) > t) --b;
+    if ((b + 1) * (b + 2) <= t) ++b;
+    *bJ = b;
+    *I = t - b * (b + 1);
+}
+
+// BLK: tile pairs (mfma4_pair) instead of single tiles, for kernels whose
+// register budget holds their two accumulators.
 template <bool TRI, bool BLK = false>
 __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows, uint32_t ws, uint32_t d,
                                                  uint32_t* ctr, unsigned long long& acc) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t nb = BLK ? (d + 63) >> 6 : (d + 31) >> 5;
-    const uint32_t ntiles = nb * (nb + 1) / 2;
+    const uint32_t nb = (d + 31) >> 5, W = (d + 63) >> 6;
+    const uint32_t ntiles = BLK ? (W - 1) * W + nb : nb * (nb + 1) / 2;
     uint32_t tile = 0;
     if (lane == 0) tile = atomicAdd(ctr, 1u);
     tile = wave_bcast(tile);
@@ -475,8 +540,13 @@ __device__ __forceinline__ void count_rows_mfma4(const unsigned long long* rows,
         uint32_t tn = 0;
         if (lane == 0) tn = atomicAdd(ctr, 1u);
         uint32_t I, J;
-        tile_coords(tile, &I, &J);
-        acc += BLK ? mfma4_block<TRI>(rows, ws, d, I, J) : mfma4_tile<TRI>(rows, ws, d, I, J);
+        if (BLK) {
+            pair_coords(tile, W, &I, &J);
+            acc += mfma4_pair<TRI>(rows, ws, d, I, J);
+        } else {
+            tile_coords(tile, &I, &J);
+            acc += mfma4_tile<TRI>(rows, ws, d, I, J);
+        }
         tile = wave_bcast(tn);
     }
 }
@@ -612,7 +682,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) root_insert(rs, L[i], uint16_t(i));
         __syncthreads();
         build_rows_block<K, NG, QU, TRI>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc, qu);
-        if (K == 4) count_rows_mfma4<TRI, false>(rows, d, d, &sTile, acc);
+        if (K == 4) count_rows_mfma4<TRI, (WPE <= 4)>(rows, d, d, &sTile, acc);
         else if (K == 5) count_rows<K>(rows, d, d, 0, 0, acc, &sTile);
         __syncthreads();
     }
