@@ -49,6 +49,7 @@ struct SQuery {
     Clock::time_point t_submit, t_start;
     kz_query_result res{};
     std::string err;  // kz_last_error() of the call that failed the query
+    bool retried = false;  // a census that failed with a device error runs once more
     bool done = false;
 };
 
@@ -387,6 +388,13 @@ void collect_aux(kz_server_s* s, std::vector<SQuery*>* gone) {
     }
     for (AuxJob& j : aux_collect(s->census)) {
         Prepared* p = j.p;
+        if (j.rc == KZ_ERR_DEVICE && !j.q->retried) {
+            // a HIP runtime error (as preparations, once more; a lasting
+            // one fails the second run too)
+            j.q->retried = true;
+            aux_submit(s->census, AuxJob{kJobCensus, j.q, p, 0, 0});
+            continue;
+        }
         if (j.rc) {
             p->busy--;
             p->preparing = false;
