@@ -164,3 +164,52 @@ def test_hub_and_bucket_paths_exact(kzdev, k):
             dg.set_buckets(k, [32, 96, 160, 224], [256, 256, 512, 1024, 512], serial)
             assert dg.count_cliques(k, "persistent") == want, (name, "mixed", serial)
         dg.release()
+
+
+_NO_QUEUE_CHILD = r"""
+import sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import openkhuzdulgraph_amd as kz
+sys.path.insert(0, sys.argv[1] + "/tests")
+from test_gpu_cliques import _oriented_by_id
+kz.init(0)
+rng = np.random.default_rng(23)
+src = np.concatenate([rng.integers(0, 3000, 60000), np.zeros(900, np.int64), np.ones(700, np.int64)])
+dst = np.concatenate([rng.integers(0, 3000, 60000), np.arange(1, 901), np.arange(2, 702)])
+graphs = {"planted_mid": _oriented_by_id(kz, kz.from_edges(src, dst, 3000)),
+          "rmat12": kz.gen_rmat(12, 16, seed=21).oriented()}
+for name, g in graphs.items():
+    dg = g.to_device()
+    print(name, *[dg.count_cliques(k, "persistent") for k in (3, 4, 5)], flush=True)
+    dg.release()
+"""
+
+
+def test_kernels_without_probe_queues_are_exact(tmp_path):
+    """KZ_STREAM_QUEUE=0 (read once per process, so in a child process):
+    the bucket kernels stream without probe queues and with full rows,
+    instances the default configuration no longer launches."""
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    import numpy as np
+
+    import openkhuzdulgraph_amd as kz
+
+    root = Path(__file__).resolve().parent.parent
+    print("[case] KZ_STREAM_QUEUE=0 child: planted_mid, rmat12, k = 3..5", flush=True)
+    out = subprocess.run([sys.executable, "-c", _NO_QUEUE_CHILD, str(root)], capture_output=True, text=True,
+                         timeout=240, env=dict(os.environ, KZ_STREAM_QUEUE="0"))
+    assert out.returncode == 0, out.stderr[-2000:]
+    got = {line.split()[0]: [int(x) for x in line.split()[1:]] for line in out.stdout.splitlines() if line.strip()}
+    rng = np.random.default_rng(23)
+    src = np.concatenate([rng.integers(0, 3000, 60000), np.zeros(900, np.int64), np.ones(700, np.int64)])
+    dst = np.concatenate([rng.integers(0, 3000, 60000), np.arange(1, 901), np.arange(2, 702)])
+    graphs = {"planted_mid": _oriented_by_id(kz, kz.from_edges(src, dst, 3000)),
+              "rmat12": kz.gen_rmat(12, 16, seed=21).oriented()}
+    for name, g in graphs.items():
+        want = [oracle.run_app(APP[k], g.rowptr, g.col)[1][0] for k in (3, 4, 5)]
+        assert got[name] == want, (name, got[name], want)
