@@ -705,7 +705,8 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
                                                        uint32_t* __restrict__ cursor,
                                                        unsigned long long* __restrict__ results,
                                                        unsigned long long* __restrict__ slab, uint32_t Dmax,
-                                                       uint32_t Wmax, uint32_t RPC, uint32_t H, uint32_t FW) {
+                                                       uint32_t Wmax, uint32_t RPC, uint32_t H, uint32_t FW,
+                                                       uint32_t fkey) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char* p = smem;
     unsigned long long* rbuf = nullptr;
@@ -758,7 +759,7 @@ __global__ __launch_bounds__(1024) void hub_rows_kernel(const QueryRef* __restri
             uint32_t fw = 0;
             rs.hmask = Hh - 1;
             rs.hshift = 32 - bits;
-            filter_geometry(d, &fw, &rs.fshift);
+            filter_geometry(d, &fw, &rs.fshift, fkey);
             const uint64_t ub = ldg(rowptr, hm.u);
             for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) L[i] = ldg(col, ub + i);
             root_clear(rs, Hh, fw, threadIdx.x, blockDim.x);
@@ -896,9 +897,9 @@ CliqueJob::~CliqueJob() {
 }
 
 // Must match the carving at the top of clique_block_kernel / hub_rows_kernel.
-static uint32_t filter_words(uint32_t Dmax) {
+static uint32_t filter_words(uint32_t Dmax, uint32_t per_key = 16) {
     uint32_t w, sh;
-    filter_geometry(Dmax, &w, &sh);
+    filter_geometry(Dmax, &w, &sh, per_key);
     return w;
 }
 
@@ -912,9 +913,9 @@ static size_t block_lds_bytes(int K, uint32_t Dmax, uint32_t W, uint32_t H, uint
 }
 
 static size_t hub_lds_bytes(int K, uint32_t Dmax, uint32_t Wmax, uint32_t RPC, uint32_t H,
-                            uint32_t queue_threads = 0) {
+                            uint32_t queue_threads = 0, uint32_t fkey = 16) {
     return (K >= 4 ? size_t(RPC) * Wmax * 8 : 0) + size_t(RPC) * 8 + size_t(queue_threads) * 8 +
-           size_t(RPC + 1) * 4 + size_t(Dmax) * 4 + size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
+           size_t(RPC + 1) * 4 + size_t(Dmax) * 4 + size_t(filter_words(Dmax, fkey)) * 4 + size_t(H) * 6 + 16;
 }
 
 // Launch shape of the hub rows kernel for row chunks of rpc rows: LDS bytes,
@@ -929,13 +930,13 @@ struct HubShape {
     uint32_t block, per_cu;
 };
 static HubShape hub_shape(int K, uint32_t dmax, uint32_t wmax, uint32_t rpc, uint32_t H, uint32_t cfg_block,
-                          bool queues) {
+                          bool queues, uint32_t fkey = 16) {
     const size_t per_block = rt().lds_per_block;
     auto fit = [&](size_t b) { return std::max<uint32_t>(1, uint32_t(per_block / (b + kStaticLdsReserve))); };
-    HubShape s{hub_lds_bytes(K, dmax, wmax, rpc, H, queues ? cfg_block : 0), cfg_block, 0};
+    HubShape s{hub_lds_bytes(K, dmax, wmax, rpc, H, queues ? cfg_block : 0, fkey), cfg_block, 0};
     s.per_cu = fit(s.lds);
     if (cfg_block == 1024) {
-        const size_t l512 = hub_lds_bytes(K, dmax, wmax, rpc, H, queues ? 512 : 0);
+        const size_t l512 = hub_lds_bytes(K, dmax, wmax, rpc, H, queues ? 512 : 0, fkey);
         if (fit(l512) >= 3) s = HubShape{l512, 512, fit(l512)};
     }
     s.per_cu = std::min<uint32_t>(s.per_cu, std::max<uint32_t>(1, kHubWavesPerCU / (s.block / 64)));
@@ -1099,7 +1100,7 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     uint32_t H, hshift;
     hash_size(hp.dmax, &H, &hshift);
     (void)hshift;
-    const HubShape sh = hub_shape(K, hp.dmax, hp.wmax, hp.rpc, H, x.cfg.block[gr.bucket], hp.queues);
+    const HubShape sh = hub_shape(K, hp.dmax, hp.wmax, hp.rpc, H, x.cfg.block[gr.bucket], hp.queues, hp.fkey);
     if (sh.lds > lds_limit)
         KZ_FAIL(KZ_ERR_UNSUPPORTED, "hub rows need %zu bytes of LDS (limit %zu)", sh.lds, lds_limit);
     const uint32_t grid = std::min<uint32_t>(hp.nitems, uint32_t(R.num_cus) * sh.per_cu);
@@ -1107,11 +1108,11 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
     if (hp.queues)
         hipLaunchKernelGGL((hub_rows_kernel<K, true>), dim3(grid), dim3(sh.block), sh.lds, s, qrefs, meta, hp.nhubs,
                            x.hub_items[K], hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax,
-                           hp.wmax, hp.rpc, H, filter_words(hp.dmax));
+                           hp.wmax, hp.rpc, H, filter_words(hp.dmax, hp.fkey), hp.fkey);
     else
         hipLaunchKernelGGL((hub_rows_kernel<K, false>), dim3(grid), dim3(sh.block), sh.lds, s, qrefs, meta, hp.nhubs,
                            x.hub_items[K], hp.nitems, cur, x.d_result, K >= 4 ? x.slabs[K] : nullptr, hp.dmax,
-                           hp.wmax, hp.rpc, H, filter_words(hp.dmax));
+                           hp.wmax, hp.rpc, H, filter_words(hp.dmax, hp.fkey), hp.fkey);
     if (K == 4) {
         const uint32_t cgrid = std::min<uint32_t>((hp.ntiles + 3) / 4, uint32_t(R.num_cus) * 8);
         hipLaunchKernelGGL(hub_mfma4_kernel, dim3(cgrid), dim3(256), 0, s, meta, hp.nhubs, x.hub_tiles[K], hp.ntiles,
@@ -1144,12 +1145,23 @@ static int prepare_hubs(CliqueJob& x, int k, const std::vector<uint32_t>& hv, co
     // holds them beside the same chunk with as many threads per CU.  Not by
     // halving the chunk: rows of 32 instead of 64 with queues measured
     // slower (RMAT-24 hub rows 17.53 -> 18.83 ms, RMAT-26/ef32 2.93 ->
-    // 3.12 s, there 16 instead of 32).
+    // 3.12 s, there 16 instead of 32).  Where the rows kernel runs one
+    // 1024-thread workgroup per CU, an 8-bit-per-id filter makes room for
+    // them (RMAT-26/ef32 hub rows 2,919 -> 2,718 ms); where it runs three
+    // 512-thread ones, that filter costs more than the queues gain (RMAT-24
+    // 17.45 -> 18.21 ms).
     if (k >= 4 && stream_queue_enabled()) {
         const uint32_t cfg_block = x.cfg.block[kNumBuckets - 1];
         const HubShape s0 = hub_shape(k, hp.dmax, hp.wmax, hp.rpc, H, cfg_block, false);
-        const HubShape sq = hub_shape(k, hp.dmax, hp.wmax, hp.rpc, H, cfg_block, true);
-        hp.queues = sq.lds <= lds_limit && sq.per_cu * sq.block >= s0.per_cu * s0.block;
+        for (uint32_t fkey : {16u, 8u}) {
+            if (fkey == 8 && !(s0.block == 1024 && s0.per_cu == 1)) break;
+            const HubShape sq = hub_shape(k, hp.dmax, hp.wmax, hp.rpc, H, cfg_block, true, fkey);
+            if (sq.lds <= lds_limit && sq.per_cu * sq.block >= s0.per_cu * s0.block) {
+                hp.queues = true;
+                hp.fkey = fkey;
+                break;
+            }
+        }
     }
     std::vector<HubMeta> meta(hp.nhubs);
     std::vector<uint32_t> items(hp.nhubs), rows(hp.nhubs), tiles(hp.nhubs);

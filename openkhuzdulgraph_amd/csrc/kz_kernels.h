@@ -269,6 +269,7 @@ struct CliqueJob {
     struct HubPlan {
         uint32_t nhubs = 0, dmax = 0, wmax = 0, rpc = 64, nitems = 0, nrows = 0, ntiles = 0;
         bool queues = false;  // rows kernel with probe queues
+        uint32_t fkey = 16;   // filter bits per id of a hub's list
     };
     std::map<int, HubPlan> hubplan;
     std::map<int, void*> hub_meta;         // HubMeta[nhubs]

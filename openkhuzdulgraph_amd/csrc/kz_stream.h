@@ -168,9 +168,11 @@ __device__ __forceinline__ void root_insert(const RootSet& s, uint32_t x, uint16
 }
 
 // Filter geometry for lists of up to dmax ids: words and shift.
-__host__ __device__ inline void filter_geometry(uint32_t dmax, uint32_t* words, uint32_t* fshift) {
+// per_key: filter bits per id of the list (16 by default).
+__host__ __device__ inline void filter_geometry(uint32_t dmax, uint32_t* words, uint32_t* fshift,
+                                                uint32_t per_key = 16) {
     uint32_t bits = 6;
-    while ((1u << bits) < 16u * dmax && bits < 31) ++bits;
+    while ((1u << bits) < per_key * dmax && bits < 31) ++bits;
     *words = (1u << bits) >> 5;
     *fshift = 32 - bits;
 }

@@ -148,6 +148,12 @@ def test_hub_and_bucket_paths_exact(kzdev, k):
     src = np.concatenate([rng.integers(0, 3000, 60000), np.zeros(900, np.int64), np.ones(700, np.int64)])
     dst = np.concatenate([rng.integers(0, 3000, 60000), np.arange(1, 901), np.arange(2, 702)])
     cases.append(("planted_mid", _oriented_by_id(kz, kz.from_edges(src, dst, 3000))))
+    # a planted hub of forward degree 5,000: its hash and list leave one
+    # 1024-thread rows workgroup per CU, which then keeps probe queues beside
+    # an 8-bit-per-id filter
+    src = np.concatenate([rng.integers(0, 6000, 60000), np.zeros(5000, np.int64)])
+    dst = np.concatenate([rng.integers(0, 6000, 60000), np.arange(1, 5001)])
+    cases.append(("planted_big_hub", _oriented_by_id(kz, kz.from_edges(src, dst, 6000))))
     for name, g in cases:
         print(f"[case] graph={name} k={k} max_degree={g.max_degree}", flush=True)
         want = oracle.run_app(APP[k], g.rowptr, g.col)[1][0]
