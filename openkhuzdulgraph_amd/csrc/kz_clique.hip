@@ -912,6 +912,24 @@ static size_t block_lds_bytes(int K, uint32_t Dmax, uint32_t W, uint32_t H, uint
            size_t(Dmax + 1) * 4 + size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
 }
 
+static void hash_size(uint32_t Dmax, uint32_t* H, uint32_t* hshift);
+
+// Largest degree in (lo, hi] whose 4-clique rows, packed, with probe queues
+// fit two 512-thread workgroups per CU while full rows fit one; 0 if none.
+static uint32_t two_per_cu_degree(uint32_t lo, uint32_t hi) {
+    const size_t per_block = rt().lds_per_block;
+    auto per_cu = [&](size_t b) { return per_block / (b + kStaticLdsReserve); };
+    for (uint32_t d = hi; d > lo; --d) {
+        uint32_t H, hshift;
+        hash_size(d, &H, &hshift);
+        const uint32_t W = (d + 63) / 64;
+        const size_t ldst = block_lds_bytes(4, d, W, H, 512, true);
+        if (ldst + kStaticLdsReserve <= per_block && per_cu(ldst) >= 2)
+            return per_cu(block_lds_bytes(4, d, W, H)) == 1 ? d : 0;
+    }
+    return 0;
+}
+
 static size_t hub_lds_bytes(int K, uint32_t Dmax, uint32_t Wmax, uint32_t RPC, uint32_t H,
                             uint32_t queue_threads = 0, uint32_t fkey = 16) {
     return (K >= 4 ? size_t(RPC) * Wmax * 8 : 0) + size_t(RPC) * 8 + size_t(queue_threads) * 8 +
@@ -1050,7 +1068,7 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         const uint32_t W = (Dmax + 63) / 64;
         uint32_t H, hshift;
         hash_size(Dmax, &H, &hshift);
-        const uint32_t block = x.cfg.block[gr.bucket];
+        const uint32_t block = gr.block ? gr.block : x.cfg.block[gr.bucket];
         const size_t lds = block_lds_bytes(K, Dmax, W, H);
         if (lds > lds_limit)
             KZ_FAIL(KZ_ERR_UNSUPPORTED, "bucket %d needs %zu bytes of LDS (limit %zu)", gr.bucket, lds, lds_limit);
@@ -1058,16 +1076,21 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
             const uint32_t n = std::max<uint32_t>(1, uint32_t(R.lds_per_block / (b + kStaticLdsReserve)));
             return std::min<uint32_t>(n, 2048 / block);
         };
-        const uint32_t per_cu = groups_per_cu(lds);
+        uint32_t per_cu = groups_per_cu(lds);
         // Probe queues where the LDS holds them at the same occupancy; for
         // 4-cliques whose full rows leave no room for them (the 512-1024
         // bucket), packed rows and queues.  Packed rows alone cost 0.4-2%
         // per bucket (RMAT-24: 128-512 72.09 -> 72.37 ms, 64-128 8.50 ->
         // 8.68), queues in the 512-1024 bucket gain 11% (66.09 -> 58.21).
+        // 512-thread groups whose full rows hold them to one per CU take
+        // packed rows where those fit two (4 waves per SIMD).
         const size_t ldsq = block_lds_bytes(K, Dmax, W, H, block);
         const size_t ldst = block_lds_bytes(K, Dmax, W, H, block, true);
         bool qu = stream_queue_enabled() && ldsq <= lds_limit && groups_per_cu(ldsq) == per_cu;
-        const bool tri = K == 4 && !qu && stream_queue_enabled() && ldst <= lds_limit && groups_per_cu(ldst) >= per_cu;
+        const bool tri_fit = K == 4 && stream_queue_enabled() && ldst <= lds_limit;
+        const bool tri2 = tri_fit && block == 512 && per_cu == 1 && groups_per_cu(ldst) >= 2;
+        const bool tri = tri2 || (tri_fit && !qu && groups_per_cu(ldst) >= per_cu);
+        if (tri2) per_cu = 2;
         const size_t lds_run = tri ? ldst : qu ? ldsq : lds;
         const uint32_t grid = std::min<uint32_t>(nt, uint32_t(R.num_cus) * per_cu);
         uint32_t FW, fshift;
@@ -1282,6 +1305,29 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         gr.count = lstart[l + 1] - lstart[l];
         const uint32_t dfirst = (*queries[hq[gr.first]].fdeg)[hv[gr.first]];  // the largest in the group
         gr.dmax = gr.bucket == 0 ? 64 : (gr.bucket == kNumBuckets - 1 ? dfirst : cfg.hi[gr.bucket]);
+        // 4-cliques, last bitmap bucket: the roots whose packed rows and
+        // probe queues fit two 512-thread workgroups per CU (degree <= the
+        // split) run as a group of their own, so one root's barriers and
+        // count overlap another root's stream on the same CU.
+        if (gr.k == 4 && gr.bucket == kNumBuckets - 2 && stream_queue_enabled()) {
+            const uint32_t split = two_per_cu_degree(cfg.hi[gr.bucket - 1], gr.dmax);
+            const uint64_t end = gr.first + gr.count;
+            uint64_t at = gr.first;  // tasks run in descending degree
+            while (split && at < end && (*queries[hq[at]].fdeg)[hv[at]] > split) ++at;
+            if (split && at < end) {
+                if (at > gr.first) {
+                    CliqueJob::Group hi = gr;
+                    hi.count = at - gr.first;
+                    hi.cursor = cursor;
+                    cursor += int(kCursorStride);
+                    x.groups.push_back(hi);
+                }
+                gr.first = at;
+                gr.count = end - at;
+                gr.dmax = split;
+                gr.block = 512;
+            }
+        }
         gr.cursor = cursor;
         // one cursor line per kernel (hubs: rows and count kernels)
         cursor += int(kCursorStride) * ((gr.bucket == kNumBuckets - 1) ? 2 : 1);

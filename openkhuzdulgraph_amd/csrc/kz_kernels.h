@@ -252,6 +252,7 @@ struct CliqueJob {
         int bucket = 0;
         uint64_t first = 0, count = 0;
         uint32_t dmax = 0;
+        uint32_t block = 0;  // workgroup size; 0: the bucket's configured one
         int cursor = 0;
     };
     bool ready = false;
