@@ -205,14 +205,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
 
+// 5-cliques through DAG rows i < j (bit j of row i set), word kj = (j+1)/64
+// on: with C = row_i & row_j (ids > j), the adjacent pairs l < m of C, i.e.
+// the sum over l in C of |C & row_l|.  C's wd <= WMAX words stay in
+// registers, so each l costs one row load per word instead of three.
+template <uint32_t WMAX>
+__device__ __forceinline__ uint32_t count5_pairs(const unsigned long long* rows, uint32_t ws, uint32_t wd,
+                                                 const unsigned long long* ri, const unsigned long long* rj,
+                                                 uint32_t kj) {
+    unsigned long long c[WMAX];
+#pragma unroll
+    for (uint32_t k = 0; k < WMAX; ++k) c[k] = (k >= kj && k < wd) ? ri[size_t(k) * ws] & rj[size_t(k) * ws] : 0ull;
+    uint32_t n = 0;  // at most |C|^2 / 2 < 2^19 for wd <= 16
+#pragma unroll
+    for (uint32_t kk = 0; kk < WMAX; ++kk) {
+        unsigned long long m = c[kk];
+        while (m) {
+            const unsigned long long* rl = rows + (kk * 64 + uint32_t(__builtin_ctzll(m)));
+            m &= m - 1;
+            // row l holds ids > l only: its words below kk are empty
+#pragma unroll
+            for (uint32_t k3 = kk; k3 < WMAX; ++k3)
+                if (k3 < wd) n += uint32_t(__popcll(c[k3] & rl[size_t(k3) * ws]));
+        }
+    }
+    return n;
+}
+
 // Counts the k-cliques of one root's DAG rows i = first, first + stride, ...
 // (wave-cooperative: lanes take the set bits j of row i).  The bitmap is
 // word-major: word kk of row i is rows[kk * ws + i], so the lanes' rows j
 // sit in consecutive words (no LDS bank conflicts) and row i is a broadcast.
 // With ctr (an LDS counter zeroed before a barrier) rows are taken one
 // at a time from it instead (first and stride unused): rows come largest
-// first, so the waves finish together.
-template <int K>
+// first, so the waves finish together.  kRegC: 5-clique rows of at most
+// 16 words keep row_i & row_j in registers (count5_pairs).
+template <int K, bool kRegC = true>
 __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t first,
                                            uint32_t stride, unsigned long long& acc, uint32_t* ctr = nullptr) {
     const uint32_t lane = threadIdx.x & 63;
@@ -243,6 +271,10 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
             const uint32_t kj = (j + 1) >> 6;
             if (K == 4) {
                 for (uint32_t kk = kj; kk < wd; ++kk) acc += __popcll(ri[size_t(kk) * ws] & rj[size_t(kk) * ws]);
+            } else if (kRegC && wd <= 16) {  // d <= 1024: C in registers
+                if (wd <= 2) acc += count5_pairs<2>(rows, ws, wd, ri, rj, kj);
+                else if (wd <= 8) acc += count5_pairs<8>(rows, ws, wd, ri, rj, kj);
+                else acc += count5_pairs<16>(rows, ws, wd, ri, rj, kj);
             } else {
                 for (uint32_t kk = kj; kk < wd; ++kk) {
                     unsigned long long m = ri[size_t(kk) * ws] & rj[size_t(kk) * ws];
@@ -807,7 +839,7 @@ __global__ __launch_bounds__(256) void hub_count_kernel(const HubMeta* __restric
             qcur = hm.q;
         }
         const uint32_t i = r - row_prefix[h];
-        count_rows<K>(slab + hm.slab_off, hm.d, hm.d, i, hm.d, acc);
+        count_rows<K, false>(slab + hm.slab_off, hm.d, hm.d, i, hm.d, acc);
     }
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
