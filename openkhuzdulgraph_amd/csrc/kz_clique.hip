@@ -291,6 +291,49 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
     }
 }
 
+// 5-cliques of one root's DAG rows in LDS (d <= 1024 rows, word-major with
+// stride d), the workgroup's lanes taking its edges (i, j) 64 at a time from
+// ctr (an LDS counter zeroed before a barrier) in row order, so that rows
+// with fewer than 64 set bits leave no lane idle.  pre: d + 1 u32 of free
+// LDS (the row prefix of edges); tmp: one u32 per wave.  Block-wide.
+__device__ __forceinline__ void count5_edges(const unsigned long long* rows, uint32_t d, uint32_t* pre,
+                                             uint32_t* tmp, uint32_t* ctr, unsigned long long& acc) {
+    const uint32_t wd = (d + 63) >> 6;
+    for (uint32_t i = threadIdx.x; i <= d; i += blockDim.x) {
+        uint32_t c = 0;
+        if (i < d)
+            for (uint32_t k = (i + 1) >> 6; k < wd; ++k) c += uint32_t(__popcll(rows[size_t(k) * d + i]));
+        pre[i] = c;
+    }
+    __syncthreads();
+    const uint32_t E = block_exclusive_scan(pre, d + 1, tmp);
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(ctr, 64u);
+    base = wave_bcast(base);
+    while (base < E) {
+        uint32_t nb = 0;
+        if (lane == 0) nb = atomicAdd(ctr, 64u);
+        const uint32_t e = base + lane;
+        if (e < E) {
+            const uint32_t i = row_in(pre, 0, d, e);  // the row holding edge e
+            const unsigned long long* ri = rows + i;
+            uint32_t rem = e - pre[i], k = (i + 1) >> 6;
+            unsigned long long wv = ri[size_t(k) * d];
+            for (uint32_t c = uint32_t(__popcll(wv)); rem >= c; c = uint32_t(__popcll(wv))) {
+                rem -= c;
+                wv = ri[size_t(++k) * d];
+            }
+            const uint32_t j = k * 64 + select_bit(wv, rem);
+            const uint32_t kj = (j + 1) >> 6;
+            if (wd <= 2) acc += count5_pairs<2>(rows, d, wd, ri, rows + j, kj);
+            else if (wd <= 8) acc += count5_pairs<8>(rows, d, wd, ri, rows + j, kj);
+            else acc += count5_pairs<16>(rows, d, wd, ri, rows + j, kj);
+        }
+        base = wave_bcast(nb);
+    }
+}
+
 // ------------------------------------------------------- MFMA 4-cliques --
 // With A the d x d 0/1 matrix of the root's DAG (A[i][j] = bit j of row i,
 // j > i), the 4-cliques at the root are sum_{i<j} A[i][j] * |row_i & row_j|
@@ -715,7 +758,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         __syncthreads();
         build_rows_block<K, NG, QU, TRI>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc, qu);
         if (K == 4) count_rows_mfma4<TRI, (WPE <= 4)>(rows, d, d, &sTile, acc);
-        else if (K == 5) count_rows<K>(rows, d, d, 0, 0, acc, &sTile);
+        else if (K == 5) {
+            if (W <= 8) count5_edges(rows, d, rp, sTmp, &sTile, acc);
+            else count_rows<K>(rows, d, d, 0, 0, acc, &sTile);
+        }
         __syncthreads();
     }
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
