@@ -205,17 +205,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
     if (qcur != 0xFFFFFFFFu) flush_query(acc, qcur, results);
 }
 
+// Word kw of row i of a word-major bitmap is rows[word_base(kw) + i]: kw *
+// ws, or, packed (TRI), 32 kw (kw + 1).  A root's DAG rows are strictly
+// upper triangular (row i has bits j > i only), so word kw of rows i >=
+// 64 (kw + 1) - 1 is always zero: the packed form keeps rows i < 64 (kw +
+// 1) of word kw (all d rows of the last), tri_words(d) words in all.
+template <bool TRI>
+__device__ __forceinline__ uint32_t word_base(uint32_t kw, uint32_t ws) {
+    return TRI ? (kw * (kw + 1)) << 5 : kw * ws;
+}
+__host__ __device__ inline uint32_t tri_words(uint32_t d) {
+    const uint32_t W = (d + 63) >> 6;
+    return W ? 32 * (W - 1) * W + d : 0u;
+}
+
 // 5-cliques through DAG rows i < j (bit j of row i set), word kj = (j+1)/64
 // on: with C = row_i & row_j (ids > j), the adjacent pairs l < m of C, i.e.
 // the sum over l in C of |C & row_l|.  C's wd <= WMAX words stay in
-// registers, so each l costs one row load per word instead of three.
-template <uint32_t WMAX>
+// registers, so each l costs one row load per word instead of three.  TRI:
+// packed rows (word_base), whose word offsets are then constants.
+template <uint32_t WMAX, bool TRI = false>
 __device__ __forceinline__ uint32_t count5_pairs(const unsigned long long* rows, uint32_t ws, uint32_t wd,
                                                  const unsigned long long* ri, const unsigned long long* rj,
                                                  uint32_t kj) {
     unsigned long long c[WMAX];
 #pragma unroll
-    for (uint32_t k = 0; k < WMAX; ++k) c[k] = (k >= kj && k < wd) ? ri[size_t(k) * ws] & rj[size_t(k) * ws] : 0ull;
+    for (uint32_t k = 0; k < WMAX; ++k)
+        c[k] = (k >= kj && k < wd) ? ri[word_base<TRI>(k, ws)] & rj[word_base<TRI>(k, ws)] : 0ull;
     uint32_t n = 0;  // at most |C|^2 / 2 < 2^19 for wd <= 16
 #pragma unroll
     for (uint32_t kk = 0; kk < WMAX; ++kk) {
@@ -226,7 +242,7 @@ __device__ __forceinline__ uint32_t count5_pairs(const unsigned long long* rows,
             // row l holds ids > l only: its words below kk are empty
 #pragma unroll
             for (uint32_t k3 = kk; k3 < WMAX; ++k3)
-                if (k3 < wd) n += uint32_t(__popcll(c[k3] & rl[size_t(k3) * ws]));
+                if (k3 < wd) n += uint32_t(__popcll(c[k3] & rl[word_base<TRI>(k3, ws)]));
         }
     }
     return n;
@@ -240,7 +256,7 @@ __device__ __forceinline__ uint32_t count5_pairs(const unsigned long long* rows,
 // at a time from it instead (first and stride unused): rows come largest
 // first, so the waves finish together.  kRegC: 5-clique rows of at most
 // 16 words keep row_i & row_j in registers (count5_pairs).
-template <int K, bool kRegC = true>
+template <int K, bool kRegC = true, bool TRI = false>
 __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint32_t ws, uint32_t d, uint32_t first,
                                            uint32_t stride, unsigned long long& acc, uint32_t* ctr = nullptr) {
     const uint32_t lane = threadIdx.x & 63;
@@ -255,35 +271,35 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
         const unsigned long long* ri = rows + i;
         const uint32_t k_lo = (i + 1) >> 6;
         uint32_t total = 0;
-        for (uint32_t k = k_lo; k < wd; ++k) total += __popcll(ri[size_t(k) * ws]);
+        for (uint32_t k = k_lo; k < wd; ++k) total += __popcll(ri[word_base<TRI>(k, ws)]);
         total = wave_bcast(total);  // row i is the same in every lane
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t r = base + lane;
             if (r >= total) break;
             uint32_t k = k_lo, rem = r;
-            unsigned long long wv = ri[size_t(k) * ws];
+            unsigned long long wv = ri[word_base<TRI>(k, ws)];
             for (uint32_t c = __popcll(wv); rem >= c; c = __popcll(wv)) {
                 rem -= c;
-                wv = ri[size_t(++k) * ws];
+                wv = ri[word_base<TRI>(++k, ws)];
             }
             const uint32_t j = k * 64 + select_bit(wv, rem);
             const unsigned long long* rj = rows + j;
             const uint32_t kj = (j + 1) >> 6;
             if (K == 4) {
-                for (uint32_t kk = kj; kk < wd; ++kk) acc += __popcll(ri[size_t(kk) * ws] & rj[size_t(kk) * ws]);
+                for (uint32_t kk = kj; kk < wd; ++kk) acc += __popcll(ri[word_base<TRI>(kk, ws)] & rj[word_base<TRI>(kk, ws)]);
             } else if (kRegC && wd <= 16) {  // d <= 1024: C in registers
-                if (wd <= 2) acc += count5_pairs<2>(rows, ws, wd, ri, rj, kj);
-                else if (wd <= 8) acc += count5_pairs<8>(rows, ws, wd, ri, rj, kj);
-                else acc += count5_pairs<16>(rows, ws, wd, ri, rj, kj);
+                if (wd <= 2) acc += count5_pairs<2, TRI>(rows, ws, wd, ri, rj, kj);
+                else if (wd <= 8) acc += count5_pairs<8, TRI>(rows, ws, wd, ri, rj, kj);
+                else acc += count5_pairs<16, TRI>(rows, ws, wd, ri, rj, kj);
             } else {
                 for (uint32_t kk = kj; kk < wd; ++kk) {
-                    unsigned long long m = ri[size_t(kk) * ws] & rj[size_t(kk) * ws];
+                    unsigned long long m = ri[word_base<TRI>(kk, ws)] & rj[word_base<TRI>(kk, ws)];
                     while (m) {
                         const uint32_t l = kk * 64 + __builtin_ctzll(m);
                         m &= m - 1;
                         const unsigned long long* rl = rows + l;
                         for (uint32_t k3 = (l + 1) >> 6; k3 < wd; ++k3)
-                            acc += __popcll(ri[size_t(k3) * ws] & rj[size_t(k3) * ws] & rl[size_t(k3) * ws]);
+                            acc += __popcll(ri[word_base<TRI>(k3, ws)] & rj[word_base<TRI>(k3, ws)] & rl[word_base<TRI>(k3, ws)]);
                     }
                 }
             }
@@ -292,17 +308,18 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
 }
 
 // 5-cliques of one root's DAG rows in LDS (d <= 1024 rows, word-major with
-// stride d), the workgroup's lanes taking its edges (i, j) 64 at a time from
+// stride d, or packed: TRI), the workgroup's lanes taking its edges (i, j) 64 at a time from
 // ctr (an LDS counter zeroed before a barrier) in row order, so that rows
 // with fewer than 64 set bits leave no lane idle.  pre: d + 1 u32 of free
 // LDS (the row prefix of edges); tmp: one u32 per wave.  Block-wide.
+template <bool TRI = false>
 __device__ __forceinline__ void count5_edges(const unsigned long long* rows, uint32_t d, uint32_t* pre,
                                              uint32_t* tmp, uint32_t* ctr, unsigned long long& acc) {
     const uint32_t wd = (d + 63) >> 6;
     for (uint32_t i = threadIdx.x; i <= d; i += blockDim.x) {
         uint32_t c = 0;
         if (i < d)
-            for (uint32_t k = (i + 1) >> 6; k < wd; ++k) c += uint32_t(__popcll(rows[size_t(k) * d + i]));
+            for (uint32_t k = (i + 1) >> 6; k < wd; ++k) c += uint32_t(__popcll(rows[word_base<TRI>(k, d) + i]));
         pre[i] = c;
     }
     __syncthreads();
@@ -319,16 +336,16 @@ __device__ __forceinline__ void count5_edges(const unsigned long long* rows, uin
             const uint32_t i = row_in(pre, 0, d, e);  // the row holding edge e
             const unsigned long long* ri = rows + i;
             uint32_t rem = e - pre[i], k = (i + 1) >> 6;
-            unsigned long long wv = ri[size_t(k) * d];
+            unsigned long long wv = ri[word_base<TRI>(k, d)];
             for (uint32_t c = uint32_t(__popcll(wv)); rem >= c; c = uint32_t(__popcll(wv))) {
                 rem -= c;
-                wv = ri[size_t(++k) * d];
+                wv = ri[word_base<TRI>(++k, d)];
             }
             const uint32_t j = k * 64 + select_bit(wv, rem);
             const uint32_t kj = (j + 1) >> 6;
-            if (wd <= 2) acc += count5_pairs<2>(rows, d, wd, ri, rows + j, kj);
-            else if (wd <= 8) acc += count5_pairs<8>(rows, d, wd, ri, rows + j, kj);
-            else acc += count5_pairs<16>(rows, d, wd, ri, rows + j, kj);
+            if (wd <= 2) acc += count5_pairs<2, TRI>(rows, d, wd, ri, rows + j, kj);
+            else if (wd <= 8) acc += count5_pairs<8, TRI>(rows, d, wd, ri, rows + j, kj);
+            else acc += count5_pairs<16, TRI>(rows, d, wd, ri, rows + j, kj);
         }
         base = wave_bcast(nb);
     }
@@ -358,20 +375,6 @@ __device__ __forceinline__ kz_v4i bits_to_bytes(uint32_t w, uint32_t off) {
     for (int q = 0; q < 4; ++q)
         r[q] = int(__umul24(__builtin_amdgcn_ubfe(w, off + 4u * uint32_t(q), 4u), 0x00204081u) & 0x01010101u);
     return r;
-}
-
-// Word kw of row i of a word-major bitmap is rows[word_base(kw) + i]: kw *
-// ws, or, packed (TRI), 32 kw (kw + 1).  A root's DAG rows are strictly
-// upper triangular (row i has bits j > i only), so word kw of rows i >=
-// 64 (kw + 1) - 1 is always zero: the packed form keeps rows i < 64 (kw +
-// 1) of word kw (all d rows of the last), tri_words(d) words in all.
-template <bool TRI>
-__device__ __forceinline__ uint32_t word_base(uint32_t kw, uint32_t ws) {
-    return TRI ? (kw * (kw + 1)) << 5 : kw * ws;
-}
-__host__ __device__ inline uint32_t tri_words(uint32_t d) {
-    const uint32_t W = (d + 63) >> 6;
-    return W ? 32 * (W - 1) * W + d : 0u;
 }
 
 // This lane's share of sum_r A[i_r][j] * c[r] over its 16 rows i_r of tile
@@ -759,8 +762,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         build_rows_block<K, NG, QU, TRI>(rowptr, col, L, d, rs, 0, d, rows, d, rb, rp, sTmp, &sSpan, acc, qu);
         if (K == 4) count_rows_mfma4<TRI, (WPE <= 4)>(rows, d, d, &sTile, acc);
         else if (K == 5) {
-            if (W <= 8) count5_edges(rows, d, rp, sTmp, &sTile, acc);
-            else count_rows<K>(rows, d, d, 0, 0, acc, &sTile);
+            if (W <= 8) count5_edges<TRI>(rows, d, rp, sTmp, &sTile, acc);
+            else count_rows<K, true, TRI>(rows, d, d, 0, 0, acc, &sTile);
         }
         __syncthreads();
     }
@@ -982,10 +985,10 @@ static uint32_t filter_words(uint32_t Dmax, uint32_t per_key = 16) {
 }
 
 // queue_threads: the workgroup size when it keeps probe queues, else 0;
-// tri: packed rows (4-cliques).
+// tri: packed rows (k >= 4).
 static size_t block_lds_bytes(int K, uint32_t Dmax, uint32_t W, uint32_t H, uint32_t queue_threads = 0,
                               bool tri = false) {
-    const size_t rows = K < 4 ? 0 : (tri && K == 4) ? size_t(tri_words(Dmax)) : size_t(Dmax) * W;
+    const size_t rows = K < 4 ? 0 : tri ? size_t(tri_words(Dmax)) : size_t(Dmax) * W;
     return rows * 8 + size_t(Dmax) * 8 + size_t(queue_threads) * 8 + size_t(Dmax) * 4 +
            size_t(Dmax + 1) * 4 + size_t(filter_words(Dmax)) * 4 + size_t(H) * 6 + 16;
 }
@@ -1059,6 +1062,10 @@ static int set_lds_attrs() {
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 2, 7, true>),
                          reinterpret_cast<const void*>(&clique_block_kernel<3, 2, 7, true>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4, kGroups, 6, true, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<5, kGroups, 6, true, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<5, 3, 4>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<5, 3, 4, true>),
+                         reinterpret_cast<const void*>(&clique_block_kernel<5, 3, 4, true, true>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 3, 4, true, true>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 4, 4, true, true>),
                          reinterpret_cast<const void*>(&clique_block_kernel<4, 2, 7, true, true>),
@@ -1097,9 +1104,9 @@ struct BlockArgs {
 template <int K, int NG, int WPE>
 static void launch_block(bool qu, bool tri, uint32_t grid, uint32_t block, size_t lds, hipStream_t s,
                          const BlockArgs& a) {
-    if constexpr (K == 4) {
+    if constexpr (K >= 4) {
         if (tri) {  // packed rows, with queues (where full rows and queues do not fit)
-            hipLaunchKernelGGL((clique_block_kernel<4, NG, WPE, true, true>), dim3(grid), dim3(block), lds, s, a.qrefs,
+            hipLaunchKernelGGL((clique_block_kernel<K, NG, WPE, true, true>), dim3(grid), dim3(block), lds, s, a.qrefs,
                                a.tv, a.tq, a.nt, a.cur, a.result, a.Dmax, a.W, a.H, a.hshift, a.FW, a.fshift);
             return;
         }
@@ -1165,7 +1172,7 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         const size_t ldsq = block_lds_bytes(K, Dmax, W, H, block);
         const size_t ldst = block_lds_bytes(K, Dmax, W, H, block, true);
         bool qu = stream_queue_enabled() && ldsq <= lds_limit && groups_per_cu(ldsq) == per_cu;
-        const bool tri_fit = K == 4 && stream_queue_enabled() && ldst <= lds_limit;
+        const bool tri_fit = K >= 4 && stream_queue_enabled() && ldst <= lds_limit;
         const bool tri2 = tri_fit && block == 512 && per_cu == 1 && groups_per_cu(ldst) >= 2;
         const bool tri = tri2 || (tri_fit && !qu && groups_per_cu(ldst) >= per_cu);
         if (tri2) per_cu = 2;
@@ -1190,6 +1197,12 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
                 // VGPRs without spills (RMAT-24 4-cliques 188.6 -> 186.1 ms)
                 if (tri) launch_block<4, 4, 4>(qu, tri, grid, block, lds_run, s, a);
                 else launch_block<4, 3, 4>(qu, tri, grid, block, lds_run, s, a);
+                return KZ_OK;
+            }
+        }
+        if constexpr (K == 5) {
+            if (per_cu * block / 64 <= 16) {  // at most 4 waves per SIMD: 128 VGPRs
+                launch_block<5, 3, 4>(qu, tri, grid, block, lds_run, s, a);
                 return KZ_OK;
             }
         }
@@ -1386,7 +1399,9 @@ int clique_job_prepare(CliqueJob& x, const std::vector<CliqueQuery>& queries, co
         // 4-cliques, last bitmap bucket: the roots whose packed rows and
         // probe queues fit two 512-thread workgroups per CU (degree <= the
         // split) run as a group of their own, so one root's barriers and
-        // count overlap another root's stream on the same CU.
+        // count overlap another root's stream on the same CU.  (5-cliques,
+        // whose count dominates, measured slower split: RMAT-22 113.8 ->
+        // 124.6 ms in that bucket.)
         if (gr.k == 4 && gr.bucket == kNumBuckets - 2 && stream_queue_enabled()) {
             const uint32_t split = two_per_cu_degree(cfg.hi[gr.bucket - 1], gr.dmax);
             const uint64_t end = gr.first + gr.count;
