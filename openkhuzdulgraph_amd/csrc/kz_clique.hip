@@ -1168,13 +1168,16 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         // per bucket (RMAT-24: 128-512 72.09 -> 72.37 ms, 64-128 8.50 ->
         // 8.68), queues in the 512-1024 bucket gain 11% (66.09 -> 58.21).
         // 512-thread groups whose full rows hold them to one per CU take
-        // packed rows where those fit two (4 waves per SIMD).
+        // packed rows where those fit two (4 waves per SIMD).  5-cliques,
+        // whose count dominates, take packed rows wherever they keep the
+        // occupancy (constant word offsets; RMAT-22 128-512 bucket 40.28 ->
+        // 39.13 ms).
         const size_t ldsq = block_lds_bytes(K, Dmax, W, H, block);
         const size_t ldst = block_lds_bytes(K, Dmax, W, H, block, true);
         bool qu = stream_queue_enabled() && ldsq <= lds_limit && groups_per_cu(ldsq) == per_cu;
         const bool tri_fit = K >= 4 && stream_queue_enabled() && ldst <= lds_limit;
         const bool tri2 = tri_fit && block == 512 && per_cu == 1 && groups_per_cu(ldst) >= 2;
-        const bool tri = tri2 || (tri_fit && !qu && groups_per_cu(ldst) >= per_cu);
+        const bool tri = tri2 || (tri_fit && (K == 5 || !qu) && groups_per_cu(ldst) >= per_cu);
         if (tri2) per_cu = 2;
         const size_t lds_run = tri ? ldst : qu ? ldsq : lds;
         const uint32_t grid = std::min<uint32_t>(nt, uint32_t(R.num_cus) * per_cu);
