@@ -308,10 +308,11 @@ __device__ __forceinline__ void count_rows(const unsigned long long* rows, uint3
 }
 
 // 5-cliques of one root's DAG rows in LDS (d <= 1024 rows, word-major with
-// stride d, or packed: TRI), the workgroup's lanes taking its edges (i, j) 64 at a time from
-// ctr (an LDS counter zeroed before a barrier) in row order, so that rows
-// with fewer than 64 set bits leave no lane idle.  pre: d + 1 u32 of free
-// LDS (the row prefix of edges); tmp: one u32 per wave.  Block-wide.
+// stride d, or packed: TRI), the workgroup's lanes taking its edges (i, j)
+// 64 at a time from ctr (an LDS counter zeroed before a barrier) in row
+// order, so that rows with fewer than 64 set bits leave no lane idle.  pre:
+// d + 1 u32 of free LDS (the row prefix of edges); tmp: one u32 per wave.
+// Block-wide.
 template <bool TRI = false>
 __device__ __forceinline__ void count5_edges(const unsigned long long* rows, uint32_t d, uint32_t* pre,
                                              uint32_t* tmp, uint32_t* ctr, unsigned long long& acc) {
@@ -1174,7 +1175,7 @@ static int launch_group(CliqueJob& x, const CliqueJob::Group& gr, hipStream_t s)
         // 39.13 ms).
         const size_t ldsq = block_lds_bytes(K, Dmax, W, H, block);
         const size_t ldst = block_lds_bytes(K, Dmax, W, H, block, true);
-        bool qu = stream_queue_enabled() && ldsq <= lds_limit && groups_per_cu(ldsq) == per_cu;
+        const bool qu = stream_queue_enabled() && ldsq <= lds_limit && groups_per_cu(ldsq) == per_cu;
         const bool tri_fit = K >= 4 && stream_queue_enabled() && ldst <= lds_limit;
         const bool tri2 = tri_fit && block == 512 && per_cu == 1 && groups_per_cu(ldst) >= 2;
         const bool tri = tri2 || (tri_fit && (K == 5 || !qu) && groups_per_cu(ldst) >= per_cu);
